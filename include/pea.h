@@ -1,0 +1,173 @@
+/*
+ * pea.h -- C ABI of libpea_hip.so, the MI355X (gfx950) spatial discretization of PyEulerALE.
+ *
+ * Replaces the f2py module `py_euler_ale.euler_ale.spatial_discretization` (the reference's FFI,
+ * src/spatial_discretization.f90:68-87, generated by meson.build:21-35).  Differences by design:
+ *   - no module globals: angle of attack and the JST constants travel in `pea_params`
+ *     (replaces the writable globals aoa, jst_k2, jst_k4, jst_c4 at f90:63-66);
+ *   - all pointers are DEVICE pointers; every call is asynchronous on `stream` (a hipStream_t,
+ *     NULL = default stream) and allocates nothing unless it says so;
+ *   - layout is structure-of-arrays with the radial index fastest (coalesced on the GPU):
+ *       states / odes       U[v*C + n*M + m]          (4 planes of M x N, C = M*N)
+ *       vertices/velocities X[d*P + n*(M+1) + m]      (2 planes of (M+1) x (N+1), P=(M+1)(N+1))
+ *       forces              F[d*N + n]
+ *     Complex arrays (directions, adjoint seeds, products) are interleaved complex128 in the
+ *     same element order (element e at doubles 2e, 2e+1).  Viewed as a tensor of shape
+ *     (4, M, N) such a plane stack has strides (C, 1, M): the reference's index (v, m, n).
+ *   - every entry point returns 0 on success or a negative pea_status; pea_last_error() gives
+ *     the message (thread-local).  M >= 4 and N >= 5 are required (the reference reads out of
+ *     range below that, f90:453-463, f90:786-803).
+ *   - output semantics are "overwrite" for every output argument.
+ */
+#ifndef PEA_H
+#define PEA_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pea_params {
+    double aoa_deg;   /* free-stream angle of attack, degrees (f90:63) */
+    double k2;        /* JST kappa_2 (f90:64), default 1     */
+    double k4;        /* JST kappa_4 (f90:65), default 1/32  */
+    double c4;        /* JST c_4     (f90:66), default 2     */
+} pea_params;
+
+enum pea_status {
+    PEA_OK = 0,
+    PEA_ERR_DIMS = -1,    /* M < 4, N < 5 or sizes overflow */
+    PEA_ERR_NULL = -2,    /* a required pointer is NULL */
+    PEA_ERR_HIP = -3,     /* a HIP runtime call failed */
+    PEA_ERR_ARG = -4      /* other invalid argument */
+};
+
+const char *pea_last_error(void);
+int pea_version(void);
+
+/* Adjoint tile-shape control for the tuner: pea_num_tiles() shapes exist; tile -1 restores the
+ * built-in choice. */
+int pea_num_tiles(void);
+int pea_set_tile_override(int tile);
+
+/* ------------------------------------------------------------------ primal */
+
+/* Fill U with the free-stream state.  Replaces set_free_stream_state (f90:471-482). */
+int pea_set_free_stream_state(const pea_params *prm, double mach, int M, int N, double *U,
+                              void *stream);
+
+/* Residual R(Ma, U, X, V), FP64.  Replaces compute_odes (f90:496-537). */
+int pea_compute_odes(const pea_params *prm, double mach, int M, int N, const double *X,
+                     const double *V, const double *U, double *R, void *stream);
+
+/* Surface forces F_n = p(U_{0,n}) normal(X_{0,n} -> X_{0,n+1}).  Replaces compute_forces
+ * (f90:1156-1174).  If coef != NULL (device, 3 doubles) it also receives the reductions
+ * (sum F_x, sum F_z, sum (z_p - zc) F_x - (x_p - xc) F_z) with x_p the face midpoints (the sums
+ * behind core.py:1277-1363); xc, zc = moment reference point. */
+int pea_compute_forces(int M, int N, const double *X, const double *U, double *F, double xc,
+                       double zc, double *coef, void *stream);
+
+/* ------------------------------------------------------------------ hand-written tangents */
+
+/* dR = dR/dMa dMa + dR/dX dX + dR/dV dV + dR/dU dU at (Ma, X, V, U): hand-written forward
+ * mode, complex directions (each may be NULL = zero; dmach is ONE complex value on the device).
+ * R (optional, may be NULL) receives the primal residual of the same pass.
+ * Replaces the products apply_odes_wrt_{mach,states,vertices}_fwd (f90:604-621, 825-864,
+ * 1005-1032) after compute_odes_wrt_* (f90:552-593, 666-750, 933-993, 1087-1143), without
+ * assembling any Jacobian. */
+int pea_jvp_odes(const pea_params *prm, double mach, int M, int N, const double *X,
+                 const double *V, const double *U, const double *dmach, const double *dX,
+                 const double *dV, const double *dU, double *dR, double *R, void *stream);
+
+/* k directions at once: dU, dX, dV are k consecutive complex arrays (stride one full array),
+ * dmach k complex values; dR receives k complex arrays.  Any direction pointer may be NULL.
+ * k in {1, 2, 4, 8}. */
+int pea_jvp_odes_multi(const pea_params *prm, double mach, int M, int N, int k,
+                       const double *X, const double *V, const double *U, const double *dmach,
+                       const double *dX, const double *dV, const double *dU, double *dR,
+                       void *stream);
+
+/* dF = dF/dX dX + dF/dU dU (complex; dX or dU may be NULL).  Replaces
+ * apply_forces_wrt_{vertices,states}_fwd (f90:1267-1288, 1333-1348). */
+int pea_jvp_forces(int M, int N, const double *X, const double *U, const double *dX,
+                   const double *dU, double *dF, void *stream);
+
+/* ------------------------------------------------------------------ hand-written adjoints */
+
+/* Transposed products with the complex seed w (4 planes, complex; NOT conjugated, as the
+ * reference's reverse products, f90:907-915): gU = (dR/dU)^T w, gX = (dR/dX)^T w,
+ * gV = (dR/dV)^T w, gmach = (dR/dMa)^T w (one complex value).  Any output may be NULL (skipped).
+ * Gather form: deterministic, no atomics; dissipation switches are recomputed.
+ * Replaces apply_odes_wrt_{states,vertices,mach}_rev (f90:875-918, 1044-1072, 632-650).
+ * Uses a workspace of pea_vjp_workspace_bytes(M, N) bytes when gmach != NULL. */
+int pea_vjp_odes(const pea_params *prm, double mach, int M, int N, const double *X,
+                 const double *V, const double *U, const double *w, double *gU, double *gX,
+                 double *gV, double *gmach, void *workspace, void *stream);
+long long pea_vjp_workspace_bytes(int M, int N);
+
+/* gU = (dF/dU)^T wF, gX = (dF/dX)^T wF (complex; either may be NULL).  Replaces
+ * apply_forces_wrt_{states,vertices}_rev (f90:1359-1375, 1299-1322). */
+int pea_vjp_forces(int M, int N, const double *X, const double *U, const double *wF,
+                   double *gU, double *gX, void *stream);
+
+/* ------------------------------------------------------------------ second order */
+
+/* Tangent-over-adjoint: with seed w and tangent direction (dmach, dX, dV, dU) (complex, NULL =
+ * zero) returns the directional derivative of the adjoint products,
+ *   hU = d/de [ (dR/dU)^T w ](Ma + e dMa, X + e dX, V + e dV, U + e dU)   and likewise hX, hV,
+ * hmach, bilinear in (w, direction) without conjugation.  The reference's second-order
+ * gradients grad_C <dA, dA/dB dB> (core.py:718-1275) are these with w = conj(dA).
+ * Workspace as pea_vjp_odes. */
+int pea_hvp_odes(const pea_params *prm, double mach, int M, int N, const double *X,
+                 const double *V, const double *U, const double *w, const double *dmach,
+                 const double *dX, const double *dV, const double *dU, double *hU,
+                 double *hX, double *hV, double *hmach, void *workspace, void *stream);
+
+/* Tangent-over-adjoint of the force adjoints: directional derivative of
+ * ((dF/dU)^T wF, (dF/dX)^T wF) along (dX, dU) (complex, NULL = zero), bilinear in (wF,
+ * direction).  The reference's grad_C <dF, dF/dB dB> (core.py:1105-1275) with wF = conj(dF). */
+int pea_hvp_forces(int M, int N, const double *X, const double *U, const double *wF,
+                   const double *dX, const double *dU, double *hU, double *hX, void *stream);
+
+/* ------------------------------------------------------------------ complex-step path */
+
+/* Directional complex step along complex directions: Im R(. + i h d)/h, two complex residual
+ * evaluations per complex direction (real and imaginary part).  h <= 0 selects 1e-12, the
+ * reference's step (f90:52). */
+int pea_cs_jvp_odes(const pea_params *prm, double mach, int M, int N, const double *X,
+                    const double *V, const double *U, const double *dmach, const double *dX,
+                    const double *dV, const double *dU, double h, double *dR, void *stream);
+
+/* Complex-step Jacobian assembly at (Ma, X, V, U), the reference's linearize (core.py:278-292):
+ *   JU  9 blocks/cell  JU[((s*4 + i)*4 + j)*C + n*M + m]  = dR_i/dU_j of neighbour s
+ *       (s = c,b,bb,f,ff,i,ii,o,oo as f90:699-707; absent radial neighbours are written 0)
+ *   JX, JV 4 blocks/cell  J[((s*4 + i)*2 + d)*C + n*M + m]  (s = bi,fi,fo,bo)
+ *   JMa  J[i*C + n*M + m]  (zero except on the far-field ring)
+ * Any output may be NULL.  Replaces compute_odes_wrt_{states,vertices,velocities,mach}. */
+int pea_cs_linearize_odes(const pea_params *prm, double mach, int M, int N, const double *X,
+                          const double *V, const double *U, double *JU, double *JX,
+                          double *JV, double *JMa, void *stream);
+
+/* Block products on the assembled Jacobians (complex vectors).  Replace
+ * apply_odes_wrt_states_fwd/rev (f90:825-918) and apply_odes_wrt_vertices_fwd/rev
+ * (f90:1005-1072, also used for the velocity blocks).  The reverse forms gather (no atomics). */
+int pea_apply_odes_wrt_states_fwd(int M, int N, const double *JU, const double *dU, double *dR,
+                                  void *stream);
+int pea_apply_odes_wrt_states_rev(int M, int N, const double *JU, const double *w, double *gU,
+                                  void *stream);
+int pea_apply_odes_wrt_vertices_fwd(int M, int N, const double *JX, const double *dX,
+                                    double *dR, void *stream);
+int pea_apply_odes_wrt_vertices_rev(int M, int N, const double *JX, const double *w,
+                                    double *gX, void *stream);
+
+/* O(nnz) export of JU to 0-based BSR (data[nnzb][4][4] row-major, int32 indices/indptr,
+ * ascending columns), nnzb = (9M-6)N, optionally subtracting the complex shift from the
+ * diagonal blocks (data then complex: 2 doubles per entry, shift_im != 0 or complex_out).
+ * Replaces convert_odes_wrt_states (f90:765-814, an all-pairs scan) + _assemble_csr
+ * (core.py:512-531). */
+int pea_export_bsr(int M, int N, const double *JU, double shift_re, double shift_im,
+                   int complex_out, double *data, int *indices, int *indptr, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PEA_H */

@@ -1,0 +1,83 @@
+"""ctypes binding of ``libpea_hip.so`` (the C ABI in ``include/pea.h``).
+
+The library is built in-tree (``pyeulerale_amd/libpea_hip.so``) by ``__graft_entry__.build()``
+or ``make -C pyeulerale_amd/csrc``.  There is no fallback: if the library is missing or cannot be
+loaded every operation raises, so a GPU run never silently takes another code path.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libpea_hip.so"
+
+_c = ctypes
+_P = _c.c_void_p
+_D = _c.c_double
+_I = _c.c_int
+
+
+class PeaParams(_c.Structure):
+    """Mirror of ``pea_params``."""
+    _fields_ = [("aoa_deg", _D), ("k2", _D), ("k4", _D), ("c4", _D)]
+
+
+# name -> argument types (restype int unless listed in _RESTYPES)
+_SIGS = {
+    "pea_last_error": [],
+    "pea_version": [],
+    "pea_set_tile_override": [_I],
+    "pea_num_tiles": [],
+    "pea_set_free_stream_state": [_P, _D, _I, _I, _P, _P],
+    "pea_compute_odes": [_P, _D, _I, _I, _P, _P, _P, _P, _P],
+    "pea_compute_forces": [_I, _I, _P, _P, _P, _D, _D, _P, _P],
+    "pea_jvp_odes": [_P, _D, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "pea_jvp_odes_multi": [_P, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "pea_jvp_forces": [_I, _I, _P, _P, _P, _P, _P, _P],
+    "pea_vjp_odes": [_P, _D, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "pea_vjp_workspace_bytes": [_I, _I],
+    "pea_vjp_forces": [_I, _I, _P, _P, _P, _P, _P, _P],
+    "pea_hvp_forces": [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P],
+    "pea_hvp_odes": [_P, _D, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "pea_cs_jvp_odes": [_P, _D, _I, _I, _P, _P, _P, _P, _P, _P, _P, _D, _P, _P],
+    "pea_cs_linearize_odes": [_P, _D, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P],
+    "pea_apply_odes_wrt_states_fwd": [_I, _I, _P, _P, _P, _P],
+    "pea_apply_odes_wrt_states_rev": [_I, _I, _P, _P, _P, _P],
+    "pea_apply_odes_wrt_vertices_fwd": [_I, _I, _P, _P, _P, _P],
+    "pea_apply_odes_wrt_vertices_rev": [_I, _I, _P, _P, _P, _P],
+    "pea_export_bsr": [_I, _I, _P, _D, _D, _I, _P, _P, _P, _P],
+}
+_RESTYPES = {"pea_last_error": _c.c_char_p, "pea_vjp_workspace_bytes": _c.c_longlong}
+
+_lib = None
+
+
+class PeaError(RuntimeError):
+    """A libpea_hip call returned a non-zero status."""
+
+
+def load() -> ctypes.CDLL:
+    """Loads and binds the library once; raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise PeaError(f"{LIB_PATH} is not built: run __graft_entry__.build() or "
+                           f"make -C {HERE / 'csrc'}")
+        lib = _c.CDLL(str(LIB_PATH))
+        for name, args in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPES.get(name, _c.c_int)
+        _lib = lib
+    return _lib
+
+
+def call(name: str, *args) -> int:
+    """Calls ``name`` and raises :class:`PeaError` on a negative status."""
+    rc = getattr(load(), name)(*args)
+    if rc < 0:
+        msg = load().pea_last_error().decode(errors="replace")
+        raise PeaError(f"{name} failed ({rc}): {msg}")
+    return rc

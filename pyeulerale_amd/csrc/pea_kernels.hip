@@ -1,0 +1,1234 @@
+// libpea_hip.so: gfx950 kernels of the PyEulerALE spatial discretization and the C ABI declared
+// in include/pea.h.
+//
+// Layout (all device memory, radial index fastest):
+//   U[v*C + n*M + m]   X[d*P + n*(M+1) + m]   complex arrays interleaved (re, im) per element.
+// Cell-centric kernels map one thread to one cell with m fastest, so neighbouring lanes touch
+// neighbouring doubles in every plane; stencil reads of the angular neighbours hit L1/L2.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "pea_adjoint.hpp"
+
+using namespace pea;
+
+// =============================================================================== errors
+namespace {
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(PEA_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define PEA_LAUNCH_CHECK(what)                                  \
+    do {                                                        \
+        hipError_t e_ = hipGetLastError();                      \
+        if (e_ != hipSuccess) return hip_fail(e_, what);        \
+    } while (0)
+
+int check_dims(int M, int N) {
+    if (M < 4 || N < 5) return fail(PEA_ERR_DIMS, "need M >= 4 and N >= 5 (got M=%d N=%d)", M, N);
+    if ((long long)(M + 1) * (N + 1) > (1LL << 30))
+        return fail(PEA_ERR_DIMS, "grid too large (M=%d N=%d)", M, N);
+    return PEA_OK;
+}
+
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+}  // namespace
+
+// =============================================================================== indexing
+struct Grid {
+    int M, N;
+    int C;   // cells
+    int P;   // vertices
+};
+
+static Grid make_grid(int M, int N) { return Grid{M, N, M * N, (M + 1) * (N + 1)}; }
+
+PEA_HD int wrapn(int n, int N) { return n < 0 ? n + N : (n >= N ? n - N : n); }
+
+// cell index (n*M + m) of stencil slot `slot` of cell (m, n); radial neighbours are only
+// requested where they exist (boundary flags)
+PEA_HD int slot_cell(const Grid& g, int m, int n, int slot) {
+    switch (slot) {
+    case B: return wrapn(n - 1, g.N) * g.M + m;
+    case BB: return wrapn(n - 2, g.N) * g.M + m;
+    case F: return wrapn(n + 1, g.N) * g.M + m;
+    case FF: return wrapn(n + 2, g.N) * g.M + m;
+    case I: return n * g.M + m - 1;
+    case II: return n * g.M + m - 2;
+    case O: return n * g.M + m + 1;
+    case OO: return n * g.M + m + 2;
+    default: return n * g.M + m;
+    }
+}
+
+// vertex index of vertex slot vtx (bi, fi, fo, bo) of cell (m, n)
+PEA_HD int vtx_index(const Grid& g, int m, int n, int vtx) {
+    const int dm = (vtx == FO || vtx == BO) ? 1 : 0;
+    const int dn = (vtx == FI || vtx == FO) ? 1 : 0;
+    return (n + dn) * (g.M + 1) + m + dm;
+}
+
+// =============================================================================== loaders
+// A loader turns device arrays into scalars of type T: states, vertices, velocities, Mach.
+
+struct LdReal {
+    const double *X, *V, *U;
+    double mach;
+    Grid g;
+    PEA_HD void state(int c, double* u) const {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) u[v] = U[v * g.C + c];
+    }
+    PEA_HD void vertex(int p, double* x) const { x[0] = X[p]; x[1] = X[g.P + p]; }
+    PEA_HD void velocity(int p, double* w) const { w[0] = V[p]; w[1] = V[g.P + p]; }
+    PEA_HD double machv() const { return mach; }
+};
+
+// K tangent lanes: lane j is the real (j even) or imaginary (j odd) part of complex direction
+// j/2; direction arrays hold K/2 consecutive complex arrays.  NULL = zero direction.
+template <int K>
+struct LdDual {
+    const double *X, *V, *U;
+    const double *dX, *dV, *dU, *dmach;
+    double mach;
+    Grid g;
+    PEA_HD void state(int c, Dual<K>* u) const {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int e = v * g.C + c;
+            u[v].v = U[e];
+#pragma unroll
+            for (int j = 0; j < K; ++j)
+                u[v].d[j] = dU ? dU[2 * ((long)(j >> 1) * 4 * g.C + e) + (j & 1)] : 0.0;
+        }
+    }
+    PEA_HD void vec2(const double* A, const double* dA, int p, Dual<K>* x) const {
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            const int e = d * g.P + p;
+            x[d].v = A[e];
+#pragma unroll
+            for (int j = 0; j < K; ++j)
+                x[d].d[j] = dA ? dA[2 * ((long)(j >> 1) * 2 * g.P + e) + (j & 1)] : 0.0;
+        }
+    }
+    PEA_HD void vertex(int p, Dual<K>* x) const { vec2(X, dX, p, x); }
+    PEA_HD void velocity(int p, Dual<K>* w) const { vec2(V, dV, p, w); }
+    PEA_HD Dual<K> machv() const {
+        Dual<K> r;
+        r.v = mach;
+#pragma unroll
+        for (int j = 0; j < K; ++j) r.d[j] = dmach ? dmach[2 * (j >> 1) + (j & 1)] : 0.0;
+        return r;
+    }
+};
+
+// Complex step along part `part` (0 real, 1 imaginary) of complex directions:
+// value + i h * part(direction).
+struct LdCx {
+    const double *X, *V, *U;
+    const double *dX, *dV, *dU, *dmach;
+    double mach, h;
+    int part;
+    Grid g;
+    PEA_HD void state(int c, Cx* u) const {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int e = v * g.C + c;
+            u[v] = Cx{U[e], dU ? h * dU[2 * e + part] : 0.0};
+        }
+    }
+    PEA_HD void vec2(const double* A, const double* dA, int p, Cx* x) const {
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            const int e = d * g.P + p;
+            x[d] = Cx{A[e], dA ? h * dA[2 * e + part] : 0.0};
+        }
+    }
+    PEA_HD void vertex(int p, Cx* x) const { vec2(X, dX, p, x); }
+    PEA_HD void velocity(int p, Cx* w) const { vec2(V, dV, p, w); }
+    PEA_HD Cx machv() const { return Cx{mach, dmach ? h * dmach[part] : 0.0}; }
+};
+
+// Stencil source of one cell over a loader.
+template <class Ld>
+struct CellSrc {
+    const Ld& ld;
+    int m, n;
+    template <class T>
+    PEA_HD void get(int slot, T* u) const {
+        ld.state(slot_cell(ld.g, m, n, slot), u);
+    }
+};
+
+template <class T, class Ld>
+PEA_HD void load_cell_geometry(const Ld& ld, int m, int n, T (*x)[2], T (*w)[2]) {
+#pragma unroll
+    for (int vtx = 0; vtx < 4; ++vtx) {
+        const int p = vtx_index(ld.g, m, n, vtx);
+        ld.vertex(p, x[vtx]);
+        ld.velocity(p, w[vtx]);
+    }
+}
+
+// =============================================================================== primal
+__global__ void __launch_bounds__(256)
+k_free_stream(pea_params P, double mach, int C, double* __restrict__ U) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    double u[4];
+    free_stream(P, mach, u);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) U[v * C + c] = u[v];
+}
+
+// One residual evaluation per thread.
+template <class T, class Ld, class Store>
+__device__ __forceinline__ void residual_cell(const pea_params& P, const Ld& ld, int c,
+                                              const Store& store) {
+    const int n = c / ld.g.M, m = c - n * ld.g.M;
+    T x[4][2], w[4][2], uc[4];
+    load_cell_geometry<T>(ld, m, n, x, w);
+    ld.state(c, uc);
+    CellGeom<T> geo;
+    cell_geometry<T>(x, w, geo);
+    T R[4];
+    cell_ode_src(P, ld.machv(), CellSrc<Ld>{ld, m, n}, uc, geo, boundary_flag(ld.g.M, m), R);
+    store(c, R);
+}
+
+struct StoreReal {
+    double* R;
+    int C;
+    PEA_HD void operator()(int c, const double* r) const {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) R[v * C + c] = r[v];
+    }
+};
+
+__global__ void __launch_bounds__(256)
+k_odes(pea_params P, LdReal ld, double* __restrict__ R) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ld.g.C) return;
+    residual_cell<double>(P, ld, c, StoreReal{R, ld.g.C});
+}
+
+// ------------------------------------------------------------------ tangent (JVP)
+template <int K>
+struct StoreDual {
+    double *dR, *R;
+    int C;
+    PEA_HD void operator()(int c, const Dual<K>* r) const {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const long e = (long)v * C + c;
+            if (R) R[e] = r[v].v;
+#pragma unroll
+            for (int j = 0; j < K; ++j) dR[2 * ((long)(j >> 1) * 4 * C + e) + (j & 1)] = r[v].d[j];
+        }
+    }
+};
+
+template <int K>
+__global__ void __launch_bounds__(256)
+k_jvp(pea_params P, LdDual<K> ld, double* __restrict__ dR, double* __restrict__ R) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ld.g.C) return;
+    residual_cell<Dual<K>>(P, ld, c, StoreDual<K>{dR, R, ld.g.C});
+}
+
+// ------------------------------------------------------------------ complex step
+struct StoreCs {
+    double* dR;
+    double h;
+    int C, part;
+    PEA_HD void operator()(int c, const Cx* r) const {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) dR[2 * ((long)v * C + c) + part] = r[v].i / h;
+    }
+};
+
+__global__ void __launch_bounds__(256)
+k_cs_jvp(pea_params P, LdCx ld, double* __restrict__ dR) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ld.g.C) return;
+    LdCx l0 = ld;
+    l0.part = 0;
+    residual_cell<Cx>(P, l0, c, StoreCs{dR, ld.h, ld.g.C, 0});
+    LdCx l1 = ld;
+    l1.part = 1;
+    residual_cell<Cx>(P, l1, c, StoreCs{dR, ld.h, ld.g.C, 1});
+}
+
+// Complex-step block assembly (the reference's linearize): one thread per cell, one complex
+// residual evaluation per perturbed input component.
+struct PertSrc {
+    const LdReal& ld;
+    int m, n, pslot, pvar;
+    double h;
+    PEA_HD void get(int slot, Cx* u) const {
+        double r[4];
+        ld.state(slot_cell(ld.g, m, n, slot), r);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) u[v] = Cx{r[v], (slot == pslot && v == pvar) ? h : 0.0};
+    }
+};
+
+__global__ void __launch_bounds__(128)
+k_cs_linearize(pea_params P, LdReal ld, double* __restrict__ JU, double* __restrict__ JX,
+               double* __restrict__ JV, double* __restrict__ JMa) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const Grid g = ld.g;
+    if (c >= g.C) return;
+    const int n = c / g.M, m = c - n * g.M;
+    const int flag = boundary_flag(g.M, m);
+    const double h = kCsStep;
+    double xr[4][2], wr[4][2], ur[4];
+    load_cell_geometry<double>(ld, m, n, xr, wr);
+    ld.state(c, ur);
+    Cx x[4][2], w[4][2], uc[4];
+    for (int a = 0; a < 4; ++a)
+        for (int d = 0; d < 2; ++d) {
+            x[a][d] = Cx{xr[a][d], 0.0};
+            w[a][d] = Cx{wr[a][d], 0.0};
+        }
+    const Cx mach = Cx{ld.mach, 0.0};
+    if (JU) {
+        const bool present[9] = {true, true, true, true, true, m > 0, m > 1, m < g.M - 1,
+                                 m < g.M - 2};
+        for (int s = 0; s < 9; ++s)
+            for (int j = 0; j < 4; ++j) {
+                double col[4] = {0.0, 0.0, 0.0, 0.0};
+                if (present[s]) {
+                    for (int v = 0; v < 4; ++v) uc[v] = Cx{ur[v], (s == C && v == j) ? h : 0.0};
+                    CellGeom<Cx> geo;
+                    cell_geometry<Cx>(x, w, geo);
+                    Cx R[4];
+                    cell_ode_src(P, mach, PertSrc{ld, m, n, s, j, h}, uc, geo, flag, R);
+                    for (int i = 0; i < 4; ++i) col[i] = R[i].i / h;
+                }
+                for (int i = 0; i < 4; ++i) JU[(long)((s * 4 + i) * 4 + j) * g.C + c] = col[i];
+            }
+    }
+    for (int v = 0; v < 4; ++v) uc[v] = Cx{ur[v], 0.0};
+    for (int which = 0; which < 2; ++which) {
+        double* J = which ? JV : JX;
+        if (!J) continue;
+        for (int a = 0; a < 4; ++a)
+            for (int d = 0; d < 2; ++d) {
+                Cx (*pt)[2] = which ? w : x;
+                pt[a][d].i = h;
+                CellGeom<Cx> geo;
+                cell_geometry<Cx>(x, w, geo);
+                Cx R[4];
+                cell_ode_src(P, mach, PertSrc{ld, m, n, -1, -1, h}, uc, geo, flag, R);
+                pt[a][d].i = 0.0;
+                for (int i = 0; i < 4; ++i) J[(long)((a * 4 + i) * 2 + d) * g.C + c] = R[i].i / h;
+            }
+    }
+    if (JMa) {
+        double col[4] = {0.0, 0.0, 0.0, 0.0};
+        if (flag == 5) {
+            CellGeom<Cx> geo;
+            cell_geometry<Cx>(x, w, geo);
+            Cx R[4];
+            cell_ode_src(P, Cx{ld.mach, h}, PertSrc{ld, m, n, -1, -1, h}, uc, geo, flag, R);
+            for (int i = 0; i < 4; ++i) col[i] = R[i].i / h;
+        }
+        for (int i = 0; i < 4; ++i) JMa[(long)i * g.C + c] = col[i];
+    }
+}
+
+// ------------------------------------------------------------------ block products
+// neighbour cell of block s of cell (m, n), -1 if absent
+PEA_HD int block_cell(const Grid& g, int m, int n, int s) {
+    switch (s) {
+    case I: return m > 0 ? slot_cell(g, m, n, s) : -1;
+    case II: return m > 1 ? slot_cell(g, m, n, s) : -1;
+    case O: return m < g.M - 1 ? slot_cell(g, m, n, s) : -1;
+    case OO: return m < g.M - 2 ? slot_cell(g, m, n, s) : -1;
+    default: return slot_cell(g, m, n, s);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_apply_states_fwd(Grid g, const double* __restrict__ J, const double* __restrict__ d,
+                   double* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.C) return;
+    const int n = c / g.M, m = c - n * g.M;
+    const int order[9] = {C, B, BB, F, FF, II, I, OO, O};   // f90:856-861
+    double ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0};
+    for (int t = 0; t < 9; ++t) {
+        const int s = order[t];
+        const int k = block_cell(g, m, n, s);
+        if (k < 0) continue;
+        double xr[4], xi[4];
+        for (int j = 0; j < 4; ++j) {
+            xr[j] = d[2 * ((long)j * g.C + k)];
+            xi[j] = d[2 * ((long)j * g.C + k) + 1];
+        }
+        for (int i = 0; i < 4; ++i) {
+            double sr = 0.0, si = 0.0;
+            for (int j = 0; j < 4; ++j) {
+                const double b = J[(long)((s * 4 + i) * 4 + j) * g.C + c];
+                sr += b * xr[j];
+                si += b * xi[j];
+            }
+            ar[i] += sr;
+            ai[i] += si;
+        }
+    }
+    for (int i = 0; i < 4; ++i) {
+        out[2 * ((long)i * g.C + c)] = ar[i];
+        out[2 * ((long)i * g.C + c) + 1] = ai[i];
+    }
+}
+
+// Gather form of the transposed block product: cell c collects block `s` of every cell k whose
+// neighbour s is c.
+__global__ void __launch_bounds__(256)
+k_apply_states_rev(Grid g, const double* __restrict__ J, const double* __restrict__ w,
+                   double* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.C) return;
+    const int n = c / g.M, m = c - n * g.M;
+    // cell k = c - offset(s) has c as its neighbour s
+    const int dm[9] = {0, 0, 0, 0, 0, -1, -2, 1, 2};
+    const int dn[9] = {0, -1, -2, 1, 2, 0, 0, 0, 0};
+    double ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0};
+    for (int s = 0; s < 9; ++s) {
+        const int mk = m - dm[s];
+        if (mk < 0 || mk >= g.M) continue;
+        const int nk = wrapn(n - dn[s], g.N);
+        const int k = nk * g.M + mk;
+        double wr[4], wi[4];
+        for (int i = 0; i < 4; ++i) {
+            wr[i] = w[2 * ((long)i * g.C + k)];
+            wi[i] = w[2 * ((long)i * g.C + k) + 1];
+        }
+        for (int j = 0; j < 4; ++j) {
+            double sr = 0.0, si = 0.0;
+            for (int i = 0; i < 4; ++i) {
+                const double b = J[(long)((s * 4 + i) * 4 + j) * g.C + k];
+                sr += wr[i] * b;
+                si += wi[i] * b;
+            }
+            ar[j] += sr;
+            ai[j] += si;
+        }
+    }
+    for (int j = 0; j < 4; ++j) {
+        out[2 * ((long)j * g.C + c)] = ar[j];
+        out[2 * ((long)j * g.C + c) + 1] = ai[j];
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_apply_vertices_fwd(Grid g, const double* __restrict__ J, const double* __restrict__ d,
+                     double* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.C) return;
+    const int n = c / g.M, m = c - n * g.M;
+    double ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0};
+    for (int a = 0; a < 4; ++a) {
+        const int p = vtx_index(g, m, n, a);
+        const double x0r = d[2 * p], x0i = d[2 * p + 1];
+        const double x1r = d[2 * ((long)g.P + p)], x1i = d[2 * ((long)g.P + p) + 1];
+        for (int i = 0; i < 4; ++i) {
+            const double b0 = J[(long)((a * 4 + i) * 2 + 0) * g.C + c];
+            const double b1 = J[(long)((a * 4 + i) * 2 + 1) * g.C + c];
+            ar[i] += b0 * x0r + b1 * x1r;
+            ai[i] += b0 * x0i + b1 * x1i;
+        }
+    }
+    for (int i = 0; i < 4; ++i) {
+        out[2 * ((long)i * g.C + c)] = ar[i];
+        out[2 * ((long)i * g.C + c) + 1] = ai[i];
+    }
+}
+
+// Gather: vertex p collects from the (up to) four cells sharing it.
+__global__ void __launch_bounds__(256)
+k_apply_vertices_rev(Grid g, const double* __restrict__ J, const double* __restrict__ w,
+                     double* __restrict__ out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.P) return;
+    const int nv = p / (g.M + 1), mv = p - nv * (g.M + 1);
+    double ar[2] = {0, 0}, ai[2] = {0, 0};
+    // cell (mv - dm, nv - dn) has this vertex as slot a
+    const int dm[4] = {0, 0, 1, 1}, dn[4] = {0, 1, 1, 0};
+    for (int a = 0; a < 4; ++a) {
+        const int mk = mv - dm[a], nk = nv - dn[a];
+        if (mk < 0 || mk >= g.M || nk < 0 || nk >= g.N) continue;
+        const int k = nk * g.M + mk;
+        for (int d = 0; d < 2; ++d) {
+            double sr = 0.0, si = 0.0;
+            for (int i = 0; i < 4; ++i) {
+                const double b = J[(long)((a * 4 + i) * 2 + d) * g.C + k];
+                sr += w[2 * ((long)i * g.C + k)] * b;
+                si += w[2 * ((long)i * g.C + k) + 1] * b;
+            }
+            ar[d] += sr;
+            ai[d] += si;
+        }
+    }
+    for (int d = 0; d < 2; ++d) {
+        out[2 * ((long)d * g.P + p)] = ar[d];
+        out[2 * ((long)d * g.P + p) + 1] = ai[d];
+    }
+}
+
+// ------------------------------------------------------------------ BSR export
+// Row `row` (cell n*M + m) lists its existing neighbours in ascending column order.
+__global__ void __launch_bounds__(256)
+k_export_bsr(Grid g, const double* __restrict__ J, double sre, double sim, int cplx,
+             double* __restrict__ data, int* __restrict__ indices, int* __restrict__ indptr) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.C) return;
+    const int n = c / g.M, m = c - n * g.M;
+    // blocks before this row: 9 per cell minus the absent radial neighbours of earlier cells
+    // (per column n: m=0 lacks i,ii; m=1 lacks ii; m=M-2 lacks oo; m=M-1 lacks o,oo)
+    auto missing_upto = [&](int mm) {   // absent blocks of cells 0..mm-1 of one column
+        int r = 0;
+        if (mm > 0) r += 2;
+        if (mm > 1) r += 1;
+        if (mm > g.M - 2) r += 1;
+        if (mm > g.M - 1) r += 2;
+        return r;
+    };
+    const long base = 9L * c - (long)n * 6 - missing_upto(m);
+    int cols[9], blks[9], cnt = 0;
+    for (int s = 0; s < 9; ++s) {
+        const int k = block_cell(g, m, n, s);
+        if (k < 0) continue;
+        cols[cnt] = k;
+        blks[cnt++] = s;
+    }
+    for (int a = 1; a < cnt; ++a)
+        for (int b = a; b > 0 && cols[b - 1] > cols[b]; --b) {
+            int t = cols[b]; cols[b] = cols[b - 1]; cols[b - 1] = t;
+            t = blks[b]; blks[b] = blks[b - 1]; blks[b - 1] = t;
+        }
+    for (int a = 0; a < cnt; ++a) {
+        const long e = base + a;
+        indices[e] = cols[a];
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) {
+                double v = J[(long)((blks[a] * 4 + i) * 4 + j) * g.C + c];
+                double vi = 0.0;
+                if (blks[a] == C && i == j) {
+                    v -= sre;
+                    vi = -sim;
+                }
+                if (cplx) {
+                    data[2 * (16 * e + 4 * i + j)] = v;
+                    data[2 * (16 * e + 4 * i + j) + 1] = vi;
+                } else {
+                    data[16 * e + 4 * i + j] = v;
+                }
+            }
+    }
+    indptr[c + 1] = (int)(base + cnt);
+    if (c == 0) indptr[0] = 0;
+}
+
+// ------------------------------------------------------------------ forces
+// One block; thread t loops over faces t, t+blockDim, ...; fixed-order LDS tree reductions.
+__global__ void __launch_bounds__(1024)
+k_forces(Grid g, const double* __restrict__ X, const double* __restrict__ U,
+         double* __restrict__ F, double xc, double zc, double* __restrict__ coef) {
+    __shared__ double red[3][1024];
+    double sx = 0.0, sz = 0.0, sm = 0.0;
+    for (int n = threadIdx.x; n < g.N; n += blockDim.x) {
+        const int a = n * (g.M + 1), b = (n + 1) * (g.M + 1);
+        double xa[2] = {X[a], X[g.P + a]}, xb[2] = {X[b], X[g.P + b]}, nr[2], u[4];
+        edge_normal(xa, xb, nr);
+        for (int v = 0; v < 4; ++v) u[v] = U[v * g.C + n * g.M];
+        const double p = pressure(u);
+        const double fx = p * nr[0], fz = p * nr[1];
+        F[n] = fx;
+        F[g.N + n] = fz;
+        const double px = (xa[0] + xb[0]) / 2.0 - xc, pz = (xa[1] + xb[1]) / 2.0 - zc;
+        sx += fx;
+        sz += fz;
+        sm += pz * fx - px * fz;
+    }
+    if (!coef) return;
+    red[0][threadIdx.x] = sx;
+    red[1][threadIdx.x] = sz;
+    red[2][threadIdx.x] = sm;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int q = 0; q < 3; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; ++q) coef[q] = red[q][0];
+}
+
+__global__ void __launch_bounds__(256)
+k_jvp_forces(Grid g, const double* __restrict__ X, const double* __restrict__ U,
+             const double* __restrict__ dX, const double* __restrict__ dU,
+             double* __restrict__ dF) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.N) return;
+    const int a = n * (g.M + 1), b = (n + 1) * (g.M + 1), c = n * g.M;
+    Dual<2> xa[2], xb[2], u[4], nr[2];
+    for (int d = 0; d < 2; ++d) {
+        xa[d].v = X[d * g.P + a];
+        xb[d].v = X[d * g.P + b];
+        for (int j = 0; j < 2; ++j) {
+            xa[d].d[j] = dX ? dX[2 * ((long)d * g.P + a) + j] : 0.0;
+            xb[d].d[j] = dX ? dX[2 * ((long)d * g.P + b) + j] : 0.0;
+        }
+    }
+    for (int v = 0; v < 4; ++v) {
+        u[v].v = U[(long)v * g.C + c];
+        for (int j = 0; j < 2; ++j) u[v].d[j] = dU ? dU[2 * ((long)v * g.C + c) + j] : 0.0;
+    }
+    edge_normal(xa, xb, nr);
+    const Dual<2> p = pressure(u);
+    for (int d = 0; d < 2; ++d) {
+        const Dual<2> f = p * nr[d];
+        dF[2 * (d * g.N + n)] = f.d[0];
+        dF[2 * (d * g.N + n) + 1] = f.d[1];
+    }
+}
+
+// Adjoint seed and output conversion for each primal type.
+template <class T>
+struct AdjIO;
+template <>
+struct AdjIO<double> {
+    static constexpr int L = 2;
+    PEA_HD static void seed(const double* w, long e, double* s) {
+        s[0] = w[2 * e];
+        s[1] = w[2 * e + 1];
+    }
+    PEA_HD static void put(double* out, long e, const double* a) {
+        out[2 * e] = a[0];
+        out[2 * e + 1] = a[1];
+    }
+};
+// tangent-over-adjoint: lanes k = real/imag part of the seed, tangent lanes j = real/imag part
+// of the direction; the complex product is bilinear in (seed, direction).
+template <>
+struct AdjIO<Dual<2>> {
+    static constexpr int L = 2;
+    PEA_HD static void seed(const double* w, long e, Dual<2>* s) {
+        s[0] = dconst<2>(w[2 * e]);
+        s[1] = dconst<2>(w[2 * e + 1]);
+    }
+    PEA_HD static void put(double* out, long e, const Dual<2>* a) {
+        out[2 * e] = a[0].d[0] - a[1].d[1];
+        out[2 * e + 1] = a[0].d[1] + a[1].d[0];
+    }
+};
+
+// Reverse of the surface forces F_n = p(U_{0,n}) normal(X_{0,n} -> X_{0,n+1}) (f90:1156-1174).
+// gU on the wall ring (zero elsewhere), one thread per cell; T = double gives the adjoint,
+// T = Dual<2> its tangent along the loader's directions.
+template <class T, class Ld>
+__global__ void __launch_bounds__(256)
+k_vjp_forces_states(Ld ld, const double* __restrict__ wF, double* __restrict__ gU) {
+    const Grid g = ld.g;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.C) return;
+    const int n = c / g.M, m = c - n * g.M;
+    T out[4][2];
+    for (int v = 0; v < 4; ++v) out[v][0] = out[v][1] = lit<T>(0.0);
+    if (m == 0) {
+        T xa[2], xb[2], nr[2], u[4];
+        ld.vertex(n * (g.M + 1), xa);
+        ld.vertex((n + 1) * (g.M + 1), xb);
+        edge_normal(xa, xb, nr);
+        ld.state(c, u);
+        const T iu = 1.0 / u[0];
+        for (int k = 0; k < 2; ++k) {
+            const T pb[1] = {wF[2 * n + k] * nr[0] + wF[2 * (g.N + n) + k] * nr[1]};
+            T ub[4][1];
+            for (int v = 0; v < 4; ++v) ub[v][0] = lit<T>(0.0);
+            pressure_rev<T, 1>(u, iu, pb, ub);
+            for (int v = 0; v < 4; ++v) out[v][k] = ub[v][0];
+        }
+    }
+    for (int v = 0; v < 4; ++v) AdjIO<T>::put(gU, (long)v * g.C + c, out[v]);
+}
+
+// gX on the surface row (zero elsewhere), one thread per vertex.
+template <class T, class Ld>
+__global__ void __launch_bounds__(256)
+k_vjp_forces_vertices(Ld ld, const double* __restrict__ wF, double* __restrict__ gX) {
+    const Grid g = ld.g;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.P) return;
+    const int nv = p / (g.M + 1), mv = p - nv * (g.M + 1);
+    T out[2][2];
+    for (int d = 0; d < 2; ++d) out[d][0] = out[d][1] = lit<T>(0.0);
+    if (mv == 0) {
+        // normal(a->b) = (b_z - a_z, a_x - b_x);  F = p n
+        for (int side = 0; side < 2; ++side) {   // 0: vertex is a of face nv, 1: b of nv-1
+            const int f = side ? nv - 1 : nv;
+            if (f < 0 || f >= g.N) continue;
+            T u[4];
+            ld.state(f * g.M, u);
+            const T pr = pressure(u);
+            const double sgn = side ? 1.0 : -1.0;
+            for (int k = 0; k < 2; ++k) {
+                const T w0 = wF[2 * f + k] * pr, w1 = wF[2 * (g.N + f) + k] * pr;
+                out[1][k] = out[1][k] + sgn * w0;    // d n0 / d z
+                out[0][k] = out[0][k] - sgn * w1;    // d n1 / d x
+            }
+        }
+    }
+    for (int d = 0; d < 2; ++d) AdjIO<T>::put(gX, (long)d * g.P + p, out[d]);
+}
+
+// =============================================================================== adjoint
+// Tile of TM x TN cells per block.  The block sweeps the reverse of every cell whose stencil
+// reaches the tile (the tile grown by two cells along both grid lines, plus the diagonal ring
+// that shares tile vertices), face by face.  Each face reverse yields four 4-vectors for the
+// cells at fixed offsets along one grid line; they are added into LDS accumulators in four
+// barrier-separated phases, so within a phase every thread writes a distinct cell.  The
+// accumulation order of every output is therefore fixed: results are bitwise deterministic
+// and independent of the tile shape.
+constexpr int kVjpThreads = 256;
+
+template <class T, int TM, int TN, bool WU, bool WX, bool WV, bool WMA, class Ld>
+__global__ void __launch_bounds__(kVjpThreads)
+k_vjp(pea_params P, Ld ld, const double* __restrict__ wseed, double* __restrict__ gU,
+      double* __restrict__ gX, double* __restrict__ gV, double* __restrict__ partial) {
+    constexpr int L = AdjIO<T>::L;
+    constexpr int BM = TM + 4, BN = TN + 4, NBOX = BM * BN;
+    constexpr int ROUNDS = (NBOX + kVjpThreads - 1) / kVjpThreads;
+    constexpr int NVX = (WX || WV) ? (TM + 1) * (TN + 1) : 1;
+    __shared__ T accU[WU ? TM * TN : 1][4][L];
+    __shared__ T accX[WX ? NVX : 1][2][L];
+    __shared__ T accV[WV ? NVX : 1][2][L];
+    __shared__ T red[WMA ? kVjpThreads : 1][L];
+
+    const Grid g = ld.g;
+    const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
+    const int tid = threadIdx.x;
+    const bool last_m = m0 + TM >= g.M, last_n = n0 + TN >= g.N;
+
+    if (WU)
+        for (int i = tid; i < TM * TN * 4 * L; i += kVjpThreads)
+            (&accU[0][0][0])[i] = lit<T>(0.0);
+    if (WX)
+        for (int i = tid; i < NVX * 2 * L; i += kVjpThreads) (&accX[0][0][0])[i] = lit<T>(0.0);
+    if (WV)
+        for (int i = tid; i < NVX * 2 * L; i += kVjpThreads) (&accV[0][0][0])[i] = lit<T>(0.0);
+    __syncthreads();
+
+    T machacc[L];
+#pragma unroll
+    for (int k = 0; k < L; ++k) machacc[k] = lit<T>(0.0);
+
+    // offsets (dm, dn) of slot j of face fc
+    constexpr int odm[4][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {1, 0, -1, -2}, {-1, 0, 1, 2}};
+    constexpr int odn[4][4] = {{1, 0, -1, -2}, {-1, 0, 1, 2}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+
+    for (int r = 0; r < ROUNDS; ++r) {
+        const int bidx = r * kVjpThreads + tid;
+        const int mb = bidx % BM, nb = bidx / BM;
+        const int m = m0 - 2 + mb;
+        const int n = wrapn(n0 - 2 + nb, g.N);
+        const bool cross = (mb >= 2 && mb < TM + 2) || (nb >= 2 && nb < TN + 2);
+        const bool ring = (WX || WV) && mb >= 1 && mb <= TM + 1 && nb >= 1 && nb <= TN + 1;
+        const bool active = bidx < NBOX && m >= 0 && m < g.M && (cross || ring);
+        const bool in_tile = active && mb >= 2 && mb < TM + 2 && nb >= 2 && nb < TN + 2 &&
+                             n0 - 2 + nb < g.N;
+        const int flag = active ? boundary_flag(g.M, m) : 3;
+        const int c = n * g.M + (active ? m : 0);
+        const CellSrc<Ld> src{ld, active ? m : 0, n};
+
+        CellAdj<T, L> ca;
+        T uc[4];
+        T x[4][2];
+        if (active) {
+            T xw[4][2];
+            load_cell_geometry<T>(ld, m, n, x, xw);
+            ld.state(c, uc);
+            T ws[4][L];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) AdjIO<T>::seed(wseed, (long)v * g.C + c, ws[v]);
+            cell_adj_init<T, L>(P, ld.machv(), src, uc, x, xw, flag, ws, WX && gX, ca);
+        }
+#pragma unroll
+        for (int fc = 0; fc < 4; ++fc) {
+            T tb[4][4][L];
+            if (active) {
+                cell_face_rev<T, L>(P, src, uc, flag, fc, ca, tb);
+                if (fc == 3) {
+                    T mb_[L];
+#pragma unroll
+                    for (int k = 0; k < L; ++k) mb_[k] = lit<T>(0.0);
+                    cell_ghost_rev<T, L>(P, ld.machv(), uc, flag, ca, tb[1], mb_);
+                    if (in_tile)
+#pragma unroll
+                        for (int k = 0; k < L; ++k) machacc[k] = machacc[k] + mb_[k];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (WU && active) {
+                    const int mt = m + odm[fc][j];
+                    const int nt = wrapn(n + odn[fc][j], g.N);
+                    if (mt >= m0 && mt < m0 + TM && mt < g.M && nt >= n0 && nt < n0 + TN) {
+                        const int t = (nt - n0) * TM + (mt - m0);
+#pragma unroll
+                        for (int v = 0; v < 4; ++v)
+#pragma unroll
+                            for (int k = 0; k < L; ++k)
+                                accU[t][v][k] = accU[t][v][k] + tb[j][v][k];
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        if (WX || WV) {
+            T xb[4][2][L], wb[4][2][L];
+            if (active) cell_vertex_rev<T, L>(x, ca, xb, wb);
+#pragma unroll
+            for (int vtx = 0; vtx < 4; ++vtx) {
+                if (active) {
+                    const int mv = m + ((vtx == FO || vtx == BO) ? 1 : 0);
+                    const int nv = n + ((vtx == FI || vtx == FO) ? 1 : 0);
+                    const bool own_m = (mv >= m0 && mv < m0 + TM) || (mv == g.M && last_m);
+                    const bool own_n = (nv >= n0 && nv < n0 + TN) || (nv == g.N && last_n);
+                    if (own_m && own_n) {
+                        const int t = (nv - n0) * (TM + 1) + (mv - m0);
+#pragma unroll
+                        for (int d = 0; d < 2; ++d)
+#pragma unroll
+                            for (int k = 0; k < L; ++k) {
+                                if (WX) accX[t][d][k] = accX[t][d][k] + xb[vtx][d][k];
+                                if (WV) accV[t][d][k] = accV[t][d][k] + wb[vtx][d][k];
+                            }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+
+    if (WU && gU)
+        for (int t = tid; t < TM * TN; t += kVjpThreads) {
+            const int mt = m0 + t % TM, nt = n0 + t / TM;
+            if (mt < g.M && nt < g.N)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    AdjIO<T>::put(gU, (long)v * g.C + nt * g.M + mt, accU[t][v]);
+        }
+    if (WX || WV)
+        for (int t = tid; t < NVX; t += kVjpThreads) {
+            const int mv = m0 + t % (TM + 1), nv = n0 + t / (TM + 1);
+            const bool own_m = (mv < m0 + TM && mv < g.M) || (mv == g.M && last_m);
+            const bool own_n = (nv < n0 + TN && nv < g.N) || (nv == g.N && last_n);
+            if (own_m && own_n)
+#pragma unroll
+                for (int d = 0; d < 2; ++d) {
+                    const long e = (long)d * g.P + nv * (g.M + 1) + mv;
+                    if (WX && gX) AdjIO<T>::put(gX, e, accX[t][d]);
+                    if (WV && gV) AdjIO<T>::put(gV, e, accV[t][d]);
+                }
+        }
+    if (WMA) {
+#pragma unroll
+        for (int k = 0; k < L; ++k) red[tid][k] = machacc[k];
+        __syncthreads();
+        for (int s = kVjpThreads / 2; s > 0; s >>= 1) {
+            if (tid < s)
+#pragma unroll
+                for (int k = 0; k < L; ++k) red[tid][k] = red[tid][k] + red[tid + s][k];
+            __syncthreads();
+        }
+        if (tid == 0 && partial)
+            AdjIO<T>::put(partial, blockIdx.y * gridDim.x + blockIdx.x, red[0]);
+    }
+}
+
+// Sums per-block complex partials in block order (deterministic).
+__global__ void __launch_bounds__(256)
+k_sum_partials(const double* __restrict__ partial, int count, double* __restrict__ out) {
+    __shared__ double red[2][256];
+    double sr = 0.0, si = 0.0;
+    for (int i = threadIdx.x; i < count; i += 256) {
+        sr += partial[2 * i];
+        si += partial[2 * i + 1];
+    }
+    red[0][threadIdx.x] = sr;
+    red[1][threadIdx.x] = si;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + s];
+            red[1][threadIdx.x] += red[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = red[0][0];
+        out[1] = red[1][0];
+    }
+}
+
+// =============================================================================== host side
+namespace {
+
+unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// cell kernels: smaller blocks when the grid is small so that all CUs get work
+int cell_block(long C) { return C >= 256L * 1024 ? 256 : 64; }
+
+struct TileShape {
+    int tm, tn;
+};
+// Tile shapes the adjoint kernels are instantiated for (tn + 4 <= N is required so a box never
+// wraps onto itself).
+constexpr TileShape kTiles[] = {{28, 12}, {12, 12}, {28, 4}, {12, 1}};
+constexpr int kNumTiles = 4;
+
+int g_tile_override = -1;
+
+// `small`: only tiles of at most 144 cells (the tangent-over-adjoint accumulators are 3x wider)
+int pick_tile(const Grid& g, bool small) {
+    if (g_tile_override >= 0 && g_tile_override < kNumTiles &&
+        kTiles[g_tile_override].tn + 4 <= g.N &&
+        (!small || kTiles[g_tile_override].tm * kTiles[g_tile_override].tn <= 144))
+        return g_tile_override;
+    // prefer the largest tile that still yields >= 512 blocks; else the one with most blocks
+    int best = -1;
+    long best_blocks = -1;
+    for (int i = 0; i < kNumTiles; ++i) {
+        if (kTiles[i].tn + 4 > g.N || (small && kTiles[i].tm * kTiles[i].tn > 144)) continue;
+        const long nb = (long)((g.M + kTiles[i].tm - 1) / kTiles[i].tm) *
+                        ((g.N + kTiles[i].tn - 1) / kTiles[i].tn);
+        if (nb >= 512) return i;
+        if (nb > best_blocks) {
+            best_blocks = nb;
+            best = i;
+        }
+    }
+    return best;
+}
+
+long tile_blocks(const Grid& g, int i) {
+    return (long)((g.M + kTiles[i].tm - 1) / kTiles[i].tm) *
+           ((g.N + kTiles[i].tn - 1) / kTiles[i].tn);
+}
+
+template <class T, int TM, int TN, class Ld>
+int launch_vjp_shape(const pea_params& P, const Ld& ld, const double* w, double* gU, double* gX,
+                     double* gV, double* partial, hipStream_t st) {
+    const dim3 grid((ld.g.M + TM - 1) / TM, (ld.g.N + TN - 1) / TN);
+    if (gU && !gX && !gV && !partial) {
+        k_vjp<T, TM, TN, true, false, false, false, Ld>
+            <<<grid, kVjpThreads, 0, st>>>(P, ld, w, gU, gX, gV, partial);
+    } else {
+        k_vjp<T, TM, TN, true, true, true, true, Ld>
+            <<<grid, kVjpThreads, 0, st>>>(P, ld, w, gU, gX, gV, partial);
+    }
+    PEA_LAUNCH_CHECK("k_vjp");
+    return PEA_OK;
+}
+
+template <class T, class Ld>
+int launch_vjp(const pea_params& P, const Ld& ld, const double* w, double* gU, double* gX,
+               double* gV, double* gmach, void* ws, hipStream_t st) {
+    constexpr bool small = sizeof(T) > sizeof(double);
+    const int tile = pick_tile(ld.g, small);
+    if (tile < 0) return fail(PEA_ERR_DIMS, "no adjoint tile fits N=%d", ld.g.N);
+    double* partial = gmach ? static_cast<double*>(ws) : nullptr;
+    if (gmach && !ws) return fail(PEA_ERR_NULL, "gmach requested without workspace");
+    int rc;
+    if constexpr (small) {
+        switch (tile) {
+        case 1: rc = launch_vjp_shape<T, 12, 12>(P, ld, w, gU, gX, gV, partial, st); break;
+        case 3: rc = launch_vjp_shape<T, 12, 1>(P, ld, w, gU, gX, gV, partial, st); break;
+        default: return fail(PEA_ERR_ARG, "tile %d not available here", tile);
+        }
+    } else {
+        switch (tile) {
+        case 0: rc = launch_vjp_shape<T, 28, 12>(P, ld, w, gU, gX, gV, partial, st); break;
+        case 1: rc = launch_vjp_shape<T, 12, 12>(P, ld, w, gU, gX, gV, partial, st); break;
+        case 2: rc = launch_vjp_shape<T, 28, 4>(P, ld, w, gU, gX, gV, partial, st); break;
+        default: rc = launch_vjp_shape<T, 12, 1>(P, ld, w, gU, gX, gV, partial, st); break;
+        }
+    }
+    if (rc) return rc;
+    if (gmach) {
+        k_sum_partials<<<1, 256, 0, st>>>(partial, (int)tile_blocks(ld.g, tile), gmach);
+        PEA_LAUNCH_CHECK("k_sum_partials");
+    }
+    return PEA_OK;
+}
+
+}  // namespace
+
+// =============================================================================== C ABI
+extern "C" {
+
+const char* pea_last_error(void) { return g_err; }
+int pea_version(void) { return 1; }
+
+int pea_set_tile_override(int tile) {
+    if (tile < -1 || tile >= kNumTiles) return fail(PEA_ERR_ARG, "tile %d out of range", tile);
+    g_tile_override = tile;
+    return PEA_OK;
+}
+int pea_num_tiles(void) { return kNumTiles; }
+
+int pea_set_free_stream_state(const pea_params* prm, double mach, int M, int N, double* U,
+                              void* stream) {
+    if (int rc = check_dims(M, N)) return rc;
+    if (!prm || !U) return fail(PEA_ERR_NULL, "null argument");
+    const int C = M * N, bs = cell_block(C);
+    k_free_stream<<<blocks_for(C, bs), bs, 0, as_stream(stream)>>>(*prm, mach, C, U);
+    PEA_LAUNCH_CHECK("k_free_stream");
+    return PEA_OK;
+}
+
+int pea_compute_odes(const pea_params* prm, double mach, int M, int N, const double* X,
+                     const double* V, const double* U, double* R, void* stream) {
+    if (int rc = check_dims(M, N)) return rc;
+    if (!prm || !X || !V || !U || !R) return fail(PEA_ERR_NULL, "null argument");
+    const Grid g = make_grid(M, N);
+    const int bs = cell_block(g.C);
+    k_odes<<<blocks_for(g.C, bs), bs, 0, as_stream(stream)>>>(*prm, LdReal{X, V, U, mach, g}, R);
+    PEA_LAUNCH_CHECK("k_odes");
+    return PEA_OK;
+}
+
+int pea_compute_forces(int M, int N, const double* X, const double* U, double* F, double xc,
+                       double zc, double* coef, void* stream) {
+    if (int rc = check_dims(M, N)) return rc;
+    if (!X || !U || !F) return fail(PEA_ERR_NULL, "null argument");
+    k_forces<<<1, 1024, 0, as_stream(stream)>>>(make_grid(M, N), X, U, F, xc, zc, coef);
+    PEA_LAUNCH_CHECK("k_forces");
+    return PEA_OK;
+}
+
+int pea_jvp_odes(const pea_params* prm, double mach, int M, int N, const double* X,
+                 const double* V, const double* U, const double* dmach, const double* dX,
+                 const double* dV, const double* dU, double* dR, double* R, void* stream) {
+    if (int rc = check_dims(M, N)) return rc;
+    if (!prm || !X || !V || !U || !dR) return fail(PEA_ERR_NULL, "null argument");
+    const Grid g = make_grid(M, N);
+    const int bs = cell_block(g.C);
+    k_jvp<2><<<blocks_for(g.C, bs), bs, 0, as_stream(stream)>>>(
+        *prm, LdDual<2>{X, V, U, dX, dV, dU, dmach, mach, g}, dR, R);
+    PEA_LAUNCH_CHECK("k_jvp");
+    return PEA_OK;
+}
+
+int pea_jvp_odes_multi(const pea_params* prm, double mach, int M, int N, int k,
+                       const double* X, const double* V, const double* U, const double* dmach,
+                       const double* dX, const double* dV, const double* dU, double* dR,
+                       void* stream) {
+    if (int rc = check_dims(M, N)) return rc;
+    if (!prm || !X || !V || !U || !dR) return fail(PEA_ERR_NULL, "null argument");
+    const Grid g = make_grid(M, N);
+    const int bs = cell_block(g.C);
+    const unsigned nb = blocks_for(g.C, bs);
+    hipStream_t st = as_stream(stream);
+    switch (k) {
+    case 1:
+        k_jvp<2><<<nb, bs, 0, st>>>(*prm, LdDual<2>{X, V, U, dX, dV, dU, dmach, mach, g}, dR,
+                                    nullptr);
+        break;
+    case 2:
+        k_jvp<4><<<nb, bs, 0, st>>>(*prm, LdDual<4>{X, V, U, dX, dV, dU, dmach, mach, g}, dR,
+                                    nullptr);
+        break;
+    case 4:
+        k_jvp<8><<<nb, bs, 0, st>>>(*prm, LdDual<8>{X, V, U, dX, dV, dU, dmach, mach, g}, dR,
+                                    nullptr);
+        break;
+    case 8:
+        for (int h = 0; h < 2; ++h) {
+            const long su = 4L * 4 * 2 * g.C, sx = 4L * 2 * 2 * g.P;
+            k_jvp<8><<<nb, bs, 0, st>>>(
+                *prm,
+                LdDual<8>{X, V, U, dX ? dX + h * sx : nullptr, dV ? dV + h * sx : nullptr,
+                          dU ? dU + h * su : nullptr, dmach ? dmach + 8 * h : nullptr, mach, g},
+                dR + h * su, nullptr);
+        }
+        break;
+    default:
+        return fail(PEA_ERR_ARG, "k must be 1, 2, 4 or 8 (got %d)", k);
+    }
+    PEA_LAUNCH_CHECK("k_jvp");
+    return PEA_OK;
+}
+
+int pea_jvp_forces(int M, int N, const double* X, const double* U, const double* dX,
+                   const double* dU, double* dF, void* stream) {
+    if (int rc = check_dims(M, N)) return rc;
+    if (!X || !U || !dF) return fail(PEA_ERR_NULL, "null argument");
+    k_jvp_forces<<<blocks_for(N, 64), 64, 0, as_stream(stream)>>>(make_grid(M, N), X, U, dX, dU,
+                                                                   dF);
+    PEA_LAUNCH_CHECK("k_jvp_forces");
+    return PEA_OK;
+}
+
+long long pea_vjp_workspace_bytes(int M, int N) {
+    if (check_dims(M, N)) return 0;
+    const Grid g = make_grid(M, N);
+    long most = 1;
+    for (int i = 0; i < kNumTiles; ++i)
+        if (kTiles[i].tn + 4 <= N && tile_blocks(g, i) > most) most = tile_blocks(g, i);
+    return (long long)most * 2 * (long long)sizeof(double);
+}
+
+int pea_vjp_odes(const pea_params* prm, double mach, int M, int N, const double* X,
+                 const double* V, const double* U, const double* w, double* gU, double* gX,
+                 double* gV, double* gmach, void* workspace, void* stream) {
+    if (int rc = check_dims(M, N)) return rc;
+    if (!prm || !X || !V || !U || !w) return fail(PEA_ERR_NULL, "null argument");
+    return launch_vjp<double>(*prm, LdReal{X, V, U, mach, make_grid(M, N)}, w, gU, gX, gV,
+                              gmach, workspace, as_stream(stream));
+}
+
+int pea_vjp_forces(int M, int N, const double* X, const double* U, const double* wF,
+                   double* gU, double* gX, void* stream) {
+    if (int rc = check_dims(M, N)) return rc;
+    if (!X || !U || !wF) return fail(PEA_ERR_NULL, "null argument");
+    const Grid g = make_grid(M, N);
+    hipStream_t st = as_stream(stream);
+    const LdReal ld{X, X, U, 0.0, g};
+    if (gU) {
+        k_vjp_forces_states<double><<<blocks_for(g.C, 256), 256, 0, st>>>(ld, wF, gU);
+        PEA_LAUNCH_CHECK("k_vjp_forces_states");
+    }
+    if (gX) {
+        k_vjp_forces_vertices<double><<<blocks_for(g.P, 256), 256, 0, st>>>(ld, wF, gX);
+        PEA_LAUNCH_CHECK("k_vjp_forces_vertices");
+    }
+    return PEA_OK;
+}
+
+int pea_hvp_forces(int M, int N, const double* X, const double* U, const double* wF,
+                   const double* dX, const double* dU, double* hU, double* hX, void* stream) {
+    if (int rc = check_dims(M, N)) return rc;
+    if (!X || !U || !wF) return fail(PEA_ERR_NULL, "null argument");
+    const Grid g = make_grid(M, N);
+    hipStream_t st = as_stream(stream);
+    const LdDual<2> ld{X, X, U, dX, nullptr, dU, nullptr, 0.0, g};
+    if (hU) {
+        k_vjp_forces_states<Dual<2>><<<blocks_for(g.C, 256), 256, 0, st>>>(ld, wF, hU);
+        PEA_LAUNCH_CHECK("k_vjp_forces_states");
+    }
+    if (hX) {
+        k_vjp_forces_vertices<Dual<2>><<<blocks_for(g.P, 256), 256, 0, st>>>(ld, wF, hX);
+        PEA_LAUNCH_CHECK("k_vjp_forces_vertices");
+    }
+    return PEA_OK;
+}
+
+int pea_hvp_odes(const pea_params* prm, double mach, int M, int N, const double* X,
+                 const double* V, const double* U, const double* w, const double* dmach,
+                 const double* dX, const double* dV, const double* dU, double* hU,
+                 double* hX, double* hV, double* hmach, void* workspace, void* stream) {
+    if (int rc = check_dims(M, N)) return rc;
+    if (!prm || !X || !V || !U || !w) return fail(PEA_ERR_NULL, "null argument");
+    return launch_vjp<Dual<2>>(*prm, LdDual<2>{X, V, U, dX, dV, dU, dmach, mach,
+                                               make_grid(M, N)},
+                               w, hU, hX, hV, hmach, workspace, as_stream(stream));
+}
+
+int pea_cs_jvp_odes(const pea_params* prm, double mach, int M, int N, const double* X,
+                    const double* V, const double* U, const double* dmach, const double* dX,
+                    const double* dV, const double* dU, double h, double* dR, void* stream) {
+    if (int rc = check_dims(M, N)) return rc;
+    if (!prm || !X || !V || !U || !dR) return fail(PEA_ERR_NULL, "null argument");
+    if (!(h > 0.0)) h = kCsStep;
+    const Grid g = make_grid(M, N);
+    const int bs = cell_block(g.C);
+    k_cs_jvp<<<blocks_for(g.C, bs), bs, 0, as_stream(stream)>>>(
+        *prm, LdCx{X, V, U, dX, dV, dU, dmach, mach, h, 0, g}, dR);
+    PEA_LAUNCH_CHECK("k_cs_jvp");
+    return PEA_OK;
+}
+
+int pea_cs_linearize_odes(const pea_params* prm, double mach, int M, int N, const double* X,
+                          const double* V, const double* U, double* JU, double* JX,
+                          double* JV, double* JMa, void* stream) {
+    if (int rc = check_dims(M, N)) return rc;
+    if (!prm || !X || !V || !U) return fail(PEA_ERR_NULL, "null argument");
+    const Grid g = make_grid(M, N);
+    k_cs_linearize<<<blocks_for(g.C, 64), 64, 0, as_stream(stream)>>>(
+        *prm, LdReal{X, V, U, mach, g}, JU, JX, JV, JMa);
+    PEA_LAUNCH_CHECK("k_cs_linearize");
+    return PEA_OK;
+}
+
+int pea_apply_odes_wrt_states_fwd(int M, int N, const double* JU, const double* dU, double* dR,
+                                  void* stream) {
+    if (int rc = check_dims(M, N)) return rc;
+    if (!JU || !dU || !dR) return fail(PEA_ERR_NULL, "null argument");
+    const Grid g = make_grid(M, N);
+    k_apply_states_fwd<<<blocks_for(g.C, 256), 256, 0, as_stream(stream)>>>(g, JU, dU, dR);
+    PEA_LAUNCH_CHECK("k_apply_states_fwd");
+    return PEA_OK;
+}
+
+int pea_apply_odes_wrt_states_rev(int M, int N, const double* JU, const double* w, double* gU,
+                                  void* stream) {
+    if (int rc = check_dims(M, N)) return rc;
+    if (!JU || !w || !gU) return fail(PEA_ERR_NULL, "null argument");
+    const Grid g = make_grid(M, N);
+    k_apply_states_rev<<<blocks_for(g.C, 256), 256, 0, as_stream(stream)>>>(g, JU, w, gU);
+    PEA_LAUNCH_CHECK("k_apply_states_rev");
+    return PEA_OK;
+}
+
+int pea_apply_odes_wrt_vertices_fwd(int M, int N, const double* JX, const double* dX,
+                                    double* dR, void* stream) {
+    if (int rc = check_dims(M, N)) return rc;
+    if (!JX || !dX || !dR) return fail(PEA_ERR_NULL, "null argument");
+    const Grid g = make_grid(M, N);
+    k_apply_vertices_fwd<<<blocks_for(g.C, 256), 256, 0, as_stream(stream)>>>(g, JX, dX, dR);
+    PEA_LAUNCH_CHECK("k_apply_vertices_fwd");
+    return PEA_OK;
+}
+
+int pea_apply_odes_wrt_vertices_rev(int M, int N, const double* JX, const double* w,
+                                    double* gX, void* stream) {
+    if (int rc = check_dims(M, N)) return rc;
+    if (!JX || !w || !gX) return fail(PEA_ERR_NULL, "null argument");
+    const Grid g = make_grid(M, N);
+    k_apply_vertices_rev<<<blocks_for(g.P, 256), 256, 0, as_stream(stream)>>>(g, JX, w, gX);
+    PEA_LAUNCH_CHECK("k_apply_vertices_rev");
+    return PEA_OK;
+}
+
+int pea_export_bsr(int M, int N, const double* JU, double shift_re, double shift_im,
+                   int complex_out, double* data, int* indices, int* indptr, void* stream) {
+    if (int rc = check_dims(M, N)) return rc;
+    if (!JU || !data || !indices || !indptr) return fail(PEA_ERR_NULL, "null argument");
+    const Grid g = make_grid(M, N);
+    const int cplx = (complex_out || shift_im != 0.0) ? 1 : 0;
+    k_export_bsr<<<blocks_for(g.C, 256), 256, 0, as_stream(stream)>>>(
+        g, JU, shift_re, shift_im, cplx, data, indices, indptr);
+    PEA_LAUNCH_CHECK("k_export_bsr");
+    return PEA_OK;
+}
+
+}  // extern "C"

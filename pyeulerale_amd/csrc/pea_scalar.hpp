@@ -1,0 +1,229 @@
+// Scalar types the discretization is instantiated on.
+//
+//   double   -- the FP64 residual and the adjoint (VJP) kernels;
+//   Cx       -- complex128 for the complex-step path (the reference's own method, whose Jacobians
+//               are Im R(u + i h e)/h, spatial_discretization.f90:45-53);
+//   Dual<K>  -- a value with K tangent lanes for the hand-written forward mode (JVP), and for
+//               tangent-over-adjoint second-order products when the adjoint runs on it.
+//
+// Every type provides the same operations, so one templated body serves all of them:
+//   +, -, *, / (with each other and with double), ssqrt, sabs, val (the real/primal part).
+// sabs carries the complex-step convention d|x| = sign(1, x) dx with a signed zero
+// (abs_c, f90:97-102); branch decisions (the biased maximum) read val() only.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#define PEA_HD __host__ __device__ inline __attribute__((always_inline))
+
+namespace pea {
+
+constexpr double kGamma = 1.4;                                   // f90:55
+constexpr double kDegToRad = 0.0174532925199432957692369;        // f90:54
+constexpr double kMaxBias = 1e-8;                                // f90:115
+constexpr double kCsStep = 1e-12;                                // f90:52
+
+// ------------------------------------------------------------------------------- double
+PEA_HD double val(double x) { return x; }
+PEA_HD double sabs(double x) { return fabs(x); }
+PEA_HD double ssqrt(double x) { return sqrt(x); }
+PEA_HD double sign1(double x) { return copysign(1.0, x); }
+
+// ------------------------------------------------------------------------------- complex
+struct Cx {
+    double r, i;
+};
+PEA_HD Cx operator+(Cx a, Cx b) { return {a.r + b.r, a.i + b.i}; }
+PEA_HD Cx operator-(Cx a, Cx b) { return {a.r - b.r, a.i - b.i}; }
+PEA_HD Cx operator-(Cx a) { return {-a.r, -a.i}; }
+PEA_HD Cx operator*(Cx a, Cx b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+PEA_HD Cx operator*(double s, Cx a) { return {s * a.r, s * a.i}; }
+PEA_HD Cx operator*(Cx a, double s) { return {a.r * s, a.i * s}; }
+PEA_HD Cx operator/(Cx a, double s) { return {a.r / s, a.i / s}; }
+PEA_HD Cx operator+(Cx a, double s) { return {a.r + s, a.i}; }
+PEA_HD Cx operator-(Cx a, double s) { return {a.r - s, a.i}; }
+PEA_HD Cx operator-(double s, Cx a) { return {s - a.r, -a.i}; }
+PEA_HD Cx operator+(double s, Cx a) { return {s + a.r, a.i}; }
+// Smith's scaled division (the range reduction Fortran compilers use for complex division).
+PEA_HD Cx operator/(Cx a, Cx b) {
+    if (fabs(b.r) >= fabs(b.i)) {
+        double t = b.i / b.r, d = b.r + b.i * t;
+        return {(a.r + a.i * t) / d, (a.i - a.r * t) / d};
+    }
+    double t = b.r / b.i, d = b.r * t + b.i;
+    return {(a.r * t + a.i) / d, (a.i * t - a.r) / d};
+}
+PEA_HD Cx operator/(double s, Cx b) { return Cx{s, 0.0} / b; }
+PEA_HD double val(Cx x) { return x.r; }
+PEA_HD Cx sabs(Cx x) { return {fabs(x.r), x.i * copysign(1.0, x.r)}; }
+// principal square root, computed without cancellation
+PEA_HD Cx ssqrt(Cx z) {
+    if (z.r == 0.0 && z.i == 0.0) return {0.0, z.i};
+    double t = sqrt(0.5 * (fabs(z.r) + hypot(z.r, z.i)));
+    if (z.r >= 0.0) return {t, z.i / (2.0 * t)};
+    return {fabs(z.i) / (2.0 * t), copysign(t, z.i)};
+}
+
+// ------------------------------------------------------------------------------- dual
+template <int K>
+struct Dual {
+    double v;
+    double d[K];
+};
+template <int K>
+PEA_HD Dual<K> dconst(double v) {
+    Dual<K> r;
+    r.v = v;
+#pragma unroll
+    for (int k = 0; k < K; ++k) r.d[k] = 0.0;
+    return r;
+}
+template <int K>
+PEA_HD Dual<K> operator+(const Dual<K>& a, const Dual<K>& b) {
+    Dual<K> r;
+    r.v = a.v + b.v;
+#pragma unroll
+    for (int k = 0; k < K; ++k) r.d[k] = a.d[k] + b.d[k];
+    return r;
+}
+template <int K>
+PEA_HD Dual<K> operator-(const Dual<K>& a, const Dual<K>& b) {
+    Dual<K> r;
+    r.v = a.v - b.v;
+#pragma unroll
+    for (int k = 0; k < K; ++k) r.d[k] = a.d[k] - b.d[k];
+    return r;
+}
+template <int K>
+PEA_HD Dual<K> operator-(const Dual<K>& a) {
+    Dual<K> r;
+    r.v = -a.v;
+#pragma unroll
+    for (int k = 0; k < K; ++k) r.d[k] = -a.d[k];
+    return r;
+}
+template <int K>
+PEA_HD Dual<K> operator*(const Dual<K>& a, const Dual<K>& b) {
+    Dual<K> r;
+    r.v = a.v * b.v;
+#pragma unroll
+    for (int k = 0; k < K; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+    return r;
+}
+template <int K>
+PEA_HD Dual<K> operator*(double s, const Dual<K>& a) {
+    Dual<K> r;
+    r.v = s * a.v;
+#pragma unroll
+    for (int k = 0; k < K; ++k) r.d[k] = s * a.d[k];
+    return r;
+}
+template <int K>
+PEA_HD Dual<K> operator*(const Dual<K>& a, double s) {
+    return s * a;
+}
+template <int K>
+PEA_HD Dual<K> operator/(const Dual<K>& a, double s) {
+    Dual<K> r;
+    r.v = a.v / s;
+#pragma unroll
+    for (int k = 0; k < K; ++k) r.d[k] = a.d[k] / s;
+    return r;
+}
+template <int K>
+PEA_HD Dual<K> operator/(const Dual<K>& a, const Dual<K>& b) {
+    Dual<K> r;
+    r.v = a.v / b.v;
+    const double inv = 1.0 / b.v;
+#pragma unroll
+    for (int k = 0; k < K; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) * inv;
+    return r;
+}
+template <int K>
+PEA_HD Dual<K> operator+(const Dual<K>& a, double s) {
+    Dual<K> r = a;
+    r.v = a.v + s;
+    return r;
+}
+template <int K>
+PEA_HD Dual<K> operator-(const Dual<K>& a, double s) {
+    Dual<K> r = a;
+    r.v = a.v - s;
+    return r;
+}
+template <int K>
+PEA_HD Dual<K> operator-(double s, const Dual<K>& a) {
+    Dual<K> r = -a;
+    r.v = s - a.v;
+    return r;
+}
+template <int K>
+PEA_HD Dual<K> operator/(double s, const Dual<K>& b) {
+    Dual<K> r;
+    r.v = s / b.v;
+    const double g = -r.v / b.v;
+#pragma unroll
+    for (int k = 0; k < K; ++k) r.d[k] = g * b.d[k];
+    return r;
+}
+template <int K>
+PEA_HD Dual<K> operator+(double s, const Dual<K>& a) {
+    return a + s;
+}
+template <int K>
+PEA_HD double val(const Dual<K>& x) {
+    return x.v;
+}
+template <int K>
+PEA_HD Dual<K> sabs(const Dual<K>& x) {
+    const double s = copysign(1.0, x.v);
+    Dual<K> r;
+    r.v = fabs(x.v);
+#pragma unroll
+    for (int k = 0; k < K; ++k) r.d[k] = s * x.d[k];
+    return r;
+}
+template <int K>
+PEA_HD Dual<K> ssqrt(const Dual<K>& x) {
+    Dual<K> r;
+    r.v = sqrt(x.v);
+    const double h = 0.5 / r.v;
+#pragma unroll
+    for (int k = 0; k < K; ++k) r.d[k] = x.d[k] * h;
+    return r;
+}
+template <int K>
+PEA_HD double sign1(const Dual<K>& x) {
+    return copysign(1.0, x.v);
+}
+
+// Constant of scalar type T.
+template <class T>
+struct Lit {
+    PEA_HD static T make(double v) { return T(v); }
+};
+template <>
+struct Lit<Cx> {
+    PEA_HD static Cx make(double v) { return {v, 0.0}; }
+};
+template <int K>
+struct Lit<Dual<K>> {
+    PEA_HD static Dual<K> make(double v) { return dconst<K>(v); }
+};
+template <class T>
+PEA_HD T lit(double v) {
+    return Lit<T>::make(v);
+}
+
+// Biased maximum: x only if val(x) exceeds val(y) by more than 1e-8, else y (f90:111-120).
+// The bias makes the choice depend on argument order; callers keep the reference's order.
+template <class T>
+PEA_HD bool maxb_takes_first(const T& x, const T& y) {
+    return val(x) > val(y) + kMaxBias;
+}
+template <class T>
+PEA_HD T maxb(const T& x, const T& y) {
+    return maxb_takes_first(x, y) ? x : y;
+}
+
+}  // namespace pea
