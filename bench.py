@@ -1,0 +1,222 @@
+#!/usr/bin/env python
+"""Benchmark of the spatial-discretization hot path on one MI355X.
+
+Workload (the metric's configuration): NACA 0004 O-grid, 257 angular x 129 radial vertices
+(M = 128 radial, N = 256 angular cells), chord 1, far field 20 chords, Ma 0.6, alpha 0 deg,
+grid rigidly pitching about the quarter chord (alpha_g = 0.5 deg sin(0.2 t) at t = 2), FP64.
+One step = one launch evaluating the residual R(Ma, U, X, V) of a batch of 64 seeded states
+U_inf (1 + 1e-3 N(0,1)) on that grid; ``value`` = residual evaluations per second.
+
+Also reported (same JSON line): per-product times of the hand-written tangent (JVP) and adjoint
+(VJP) against the GPU complex-step comparators (directional complex step, and the reference's
+assembled-block products), their errors, the same at 1025 x 513, the roofline of the dominant
+kernel measured with HIP events on its stream, and the reference's own Fortran (compiled from
+its source into oracle/_ref) timed on this host's cores as the CPU baseline.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+BATCH = 64
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-large", action="store_true", help="skip the 1025x513 products")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    return ap.parse_args()
+
+
+def make_case(A, Rv, thickness, mach, aoa, alpha, alpha_dot, far, batch, seed=0):
+    from pyeulerale_amd.grid import naca_ogrid, pitch
+    X0 = naca_ogrid(A, Rv, thickness, far_field=far)
+    X, V = pitch(X0, alpha, alpha_dot)
+    M, N = Rv - 1, A - 1
+    a = np.deg2rad(aoa)
+    s = mach * np.sqrt(1.4)
+    uinf = np.array([1.0, s * np.cos(a), s * np.sin(a), 1.0 / 0.4 + s * s / 2.0])
+    rng = np.random.default_rng(seed)
+    U = uinf[None, :, None, None] * (1.0 + 1e-3 * rng.standard_normal((batch, 4, M, N)))
+    return X, V, U, M, N
+
+
+def graph_time_ms(torch, fn, reps=20):
+    """Average GPU time of fn() over `reps` launches captured in one graph (no host gaps)."""
+    fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        for _ in range(2):
+            fn()
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        for _ in range(reps):
+            fn()
+    g.replay()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(3):
+        g.replay()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / (3 * reps)
+
+
+def rel(a, b):
+    import torch
+    return float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+
+
+def products(torch, K, A, Rv, thickness, mach, aoa, alpha, alpha_dot, far, reps):
+    """Hand-written JVP/VJP vs complex-step comparators on one grid."""
+    X, V, U, M, N = make_case(A, Rv, thickness, mach, aoa, alpha, alpha_dot, far, 1)
+    g, p = K.Grid(M, N), K.Params(aoa_deg=aoa)
+    dev = lambda a, dt: torch.as_tensor(np.ascontiguousarray(np.swapaxes(a, -1, -2))).to(dt).cuda()
+    Xd, Vd, Ud = dev(X, torch.float64), dev(V, torch.float64), dev(U[0], torch.float64)
+    rng = np.random.default_rng(1)
+    cshape = (4, M, N)
+    dU = dev(rng.standard_normal(cshape) + 1j * rng.standard_normal(cshape), torch.complex128)
+    w = dev(rng.standard_normal(cshape) + 1j * rng.standard_normal(cshape), torch.complex128)
+    out = torch.empty_like(dU)
+    ws = K._workspace(g, Ud.device)
+    gU = torch.empty_like(dU)
+    JU, _, _, _ = K.cs_linearize_odes(p, mach, g, Xd, Vd, Ud, want=("states",))
+    t = {}
+    t["jvp"] = graph_time_ms(torch, lambda: K.jvp_odes(p, mach, g, Xd, Vd, Ud, dU=dU, dR=out),
+                             reps)
+    t["vjp"] = graph_time_ms(torch, lambda: K._lib.call(
+        "pea_vjp_odes", K.ctypes.byref(p.c()), float(mach), g.M, g.N, K._ptr(Xd), K._ptr(Vd),
+        K._ptr(Ud), K._ptr(w), K._ptr(gU), None, None, None, K._ptr(ws), K._stream()), reps)
+    t["cs_dir"] = graph_time_ms(torch, lambda: K.cs_jvp_odes(p, mach, g, Xd, Vd, Ud, dU=dU,
+                                                             dR=out), reps)
+    t["cs_asm_apply_fwd"] = graph_time_ms(torch, lambda: K.apply_states_fwd(g, JU, dU, out), reps)
+    t["cs_asm_apply_rev"] = graph_time_ms(torch, lambda: K.apply_states_rev(g, JU, w, out), reps)
+    t["cs_linearize"] = graph_time_ms(torch, lambda: K.cs_linearize_odes(
+        p, mach, g, Xd, Vd, Ud, JU=JU, want=("states",)), max(2, reps // 10))
+    t["residual"] = graph_time_ms(torch, lambda: K.compute_odes(p, mach, g, Xd, Vd, Ud), reps)
+    # accuracy of the hand-written products against complex step (h = 1e-12)
+    jvp = K.jvp_odes(p, mach, g, Xd, Vd, Ud, dU=dU)
+    csd = K.cs_jvp_odes(p, mach, g, Xd, Vd, Ud, dU=dU)
+    vjp = K.vjp_odes(p, mach, g, Xd, Vd, Ud, w)["states"]
+    asm = K.apply_states_rev(g, JU, w)
+    C, P = M * N, (M + 1) * (N + 1)
+    res = {
+        "grid": f"{A}x{Rv}", "cells": C,
+        "ms": {k: round(v, 5) for k, v in t.items()},
+        "rel_err_jvp_vs_cs": rel(jvp, csd), "rel_err_vjp_vs_cs_blocks": rel(vjp, asm),
+        "speedup_vjp_vs_cs_dir": t["cs_dir"] / t["vjp"],
+        "speedup_jvp_vs_cs_dir": t["cs_dir"] / t["jvp"],
+        "speedup_vjp_vs_cs_asm_rev": t["cs_asm_apply_rev"] / t["vjp"],
+        "speedup_vjp_vs_cs_linearize_plus_rev": (t["cs_linearize"] + t["cs_asm_apply_rev"]) / t["vjp"],
+        "jvp_hbm_frac": (160 * C + 32 * P) / (t["jvp"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "vjp_hbm_frac": (160 * C + 32 * P) / (t["vjp"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+    }
+    return res
+
+
+def cpu_baseline(seconds, X, V, U, mach, aoa):
+    """The reference's compiled Fortran compute_odes on one host core (bounded sample)."""
+    from oracle.oracle import Oracle, Params
+    kind = "reference" if Oracle.available("ref") else "port"
+    orc = Oracle("ref" if kind == "reference" else "port")
+    prm = Params(aoa=aoa)
+    Xc, Vc = X.astype(complex), V.astype(complex)
+    Uc = [np.asfortranarray(U[i].astype(complex)) for i in range(U.shape[0])]
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        orc.compute_odes(prm, mach, Xc, Vc, Uc[n % len(Uc)])
+        n += 1
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "residual evals/s", "cores": 1, "kind": kind,
+            "sample": f"{n} single-state residual evaluations of the 257x129 workload "
+                      f"({'compiled reference Fortran, f90 compute_odes' if kind == 'reference' else 'C restatement'}) "
+                      f"on 1 host core in {dt:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus N>1 is not supported",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+
+    from pyeulerale_amd import kernels as K
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    mach, aoa = 0.6, 0.0
+    omega, t_ = 0.2, 2.0
+    amp = np.deg2rad(0.5)
+    alpha, alpha_dot = amp * np.sin(omega * t_), amp * omega * np.cos(omega * t_)
+    X, V, U, M, N = make_case(257, 129, 0.04, mach, aoa, alpha, alpha_dot, 20.0, BATCH)
+    g, p = K.Grid(M, N), K.Params(aoa_deg=aoa)
+    Xd = torch.as_tensor(np.ascontiguousarray(np.swapaxes(X, -1, -2))).cuda()
+    Vd = torch.as_tensor(np.ascontiguousarray(np.swapaxes(V, -1, -2))).cuda()
+    Ud = torch.as_tensor(np.ascontiguousarray(np.swapaxes(U, -1, -2))).cuda()
+    Rd = torch.empty_like(Ud)
+    step = lambda: K.compute_odes_batch(p, mach, g, Xd, Vd, Ud, Rd)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for a, b in evs:
+        a.record()
+        step()
+        b.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    C, P = M * N, (M + 1) * (N + 1)
+    bytes_launch = BATCH * 64 * C + 32 * P
+    achieved = bytes_launch / (kern_ms * 1e-3) / 1e9
+    value = BATCH * args.steps / wall
+
+    extra = {"products_257x129": products(torch, K, 257, 129, 0.04, mach, aoa, alpha,
+                                          alpha_dot, 20.0, 50)}
+    if not args.no_large:
+        extra["products_1025x513"] = products(torch, K, 1025, 513, 0.04, 0.5, 0.0, 0.0, 0.0,
+                                              30.0, 10)
+    line = {
+        "metric": "residual evals/s + hand-written VJP vs complex-step JVP speedup (FP64, 1 GPU)",
+        "value": value, "unit": "residual evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "NACA0004 O-grid 257x129 vertices (M=128, N=256), pitching, "
+                               "Ma 0.6, batch of 64 states per launch",
+                   "batch": BATCH, "grid": "257x129"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_odes", "kernel_ms": kern_ms,
+                     "algorithmic_bytes_per_launch": bytes_launch},
+        "speedup_vjp_vs_cs_jvp": extra["products_257x129"]["speedup_vjp_vs_cs_dir"],
+        "extra": extra,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds, X, V, U[:4], mach, aoa)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -59,6 +59,12 @@ int pea_set_free_stream_state(const pea_params *prm, double mach, int M, int N, 
 int pea_compute_odes(const pea_params *prm, double mach, int M, int N, const double *X,
                      const double *V, const double *U, double *R, void *stream);
 
+/* Residuals of `nstates` states on one grid in one launch: U and R hold nstates consecutive
+ * state arrays (stride 4*M*N doubles).  Same arithmetic as pea_compute_odes. */
+int pea_compute_odes_batch(const pea_params *prm, double mach, int M, int N, int nstates,
+                           const double *X, const double *V, const double *U, double *R,
+                           void *stream);
+
 /* Surface forces F_n = p(U_{0,n}) normal(X_{0,n} -> X_{0,n+1}).  Replaces compute_forces
  * (f90:1156-1174).  If coef != NULL (device, 3 doubles) it also receives the reductions
  * (sum F_x, sum F_z, sum (z_p - zc) F_x - (x_p - xc) F_z) with x_p the face midpoints (the sums

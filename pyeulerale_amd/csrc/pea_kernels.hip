@@ -219,11 +219,15 @@ struct StoreReal {
     }
 };
 
+// blockIdx.y selects one of several states on the same grid (consecutive 4*C arrays)
 __global__ void __launch_bounds__(256)
 k_odes(pea_params P, LdReal ld, double* __restrict__ R) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ld.g.C) return;
-    residual_cell<double>(P, ld, c, StoreReal{R, ld.g.C});
+    const long off = (long)blockIdx.y * 4 * ld.g.C;
+    LdReal l = ld;
+    l.U = ld.U + off;
+    residual_cell<double>(P, l, c, StoreReal{R + off, ld.g.C});
 }
 
 // ------------------------------------------------------------------ tangent (JVP)
@@ -1012,6 +1016,20 @@ int pea_compute_odes(const pea_params* prm, double mach, int M, int N, const dou
     const Grid g = make_grid(M, N);
     const int bs = cell_block(g.C);
     k_odes<<<blocks_for(g.C, bs), bs, 0, as_stream(stream)>>>(*prm, LdReal{X, V, U, mach, g}, R);
+    PEA_LAUNCH_CHECK("k_odes");
+    return PEA_OK;
+}
+
+int pea_compute_odes_batch(const pea_params* prm, double mach, int M, int N, int nstates,
+                           const double* X, const double* V, const double* U, double* R,
+                           void* stream) {
+    if (int rc = check_dims(M, N)) return rc;
+    if (!prm || !X || !V || !U || !R) return fail(PEA_ERR_NULL, "null argument");
+    if (nstates < 1 || nstates > 65535) return fail(PEA_ERR_ARG, "nstates %d", nstates);
+    const Grid g = make_grid(M, N);
+    const int bs = cell_block((long)g.C * nstates);
+    k_odes<<<dim3(blocks_for(g.C, bs), nstates), bs, 0, as_stream(stream)>>>(
+        *prm, LdReal{X, V, U, mach, g}, R);
     PEA_LAUNCH_CHECK("k_odes");
     return PEA_OK;
 }

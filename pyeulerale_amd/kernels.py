@@ -119,6 +119,18 @@ def compute_odes(p: Params, mach: float, g: Grid, X, V, U, R=None) -> torch.Tens
     return R
 
 
+def compute_odes_batch(p: Params, mach: float, g: Grid, X, V, U, R=None) -> torch.Tensor:
+    """Residuals of a batch of states ``U`` of shape ``(B, 4, N, M)`` on one grid."""
+    _need(X, g.vert_shape, F64, "vertices")
+    _need(V, g.vert_shape, F64, "velocities")
+    B = U.shape[0]
+    _need(U, (B, *g.cell_shape), F64, "states")
+    R = torch.empty_like(U) if R is None else _need(R, (B, *g.cell_shape), F64, "odes")
+    _lib.call("pea_compute_odes_batch", ctypes.byref(p.c()), float(mach), g.M, g.N, B, _ptr(X),
+              _ptr(V), _ptr(U), _ptr(R), _stream())
+    return R
+
+
 def compute_forces(g: Grid, X, U, F=None, center=(0.0, 0.0), coef=None):
     _need(X, g.vert_shape, F64, "vertices")
     _need(U, g.cell_shape, F64, "states")
