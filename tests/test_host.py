@@ -1,0 +1,108 @@
+"""CPU tests of the host-side pieces: grid generation, PLOT3D I/O, pitching kinematics, the
+C ABI library (loads and exports every symbol include/pea.h declares) and argument checks."""
+
+from __future__ import annotations
+
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from pyeulerale_amd.grid import naca_ogrid, pitch, read_plot3d, rotation_gain, write_plot3d
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _areas(X):
+    d1 = X[:, 1:, 1:] - X[:, :-1, :-1]
+    d2 = X[:, 1:, :-1] - X[:, :-1, 1:]
+    return 0.5 * (d1[0] * d2[1] - d1[1] * d2[0])
+
+
+@pytest.mark.parametrize("A,Rv,t", [(129, 65, 0.12), (257, 129, 0.04), (33, 9, 0.18)])
+def test_ogrid_properties(A, Rv, t):
+    X = naca_ogrid(A, Rv, t, far_field=20.0)
+    assert X.shape == (2, Rv, A)
+    np.testing.assert_array_equal(X[:, :, 0], X[:, :, -1])           # closed O-grid
+    assert np.all(_areas(X) > 0)                                       # CCW cells
+    np.testing.assert_allclose(X[:, 0, 0], [1.0, 0.0], atol=1e-15)     # starts at the TE
+    assert X[1, 0, 1] < 0                                              # then the lower side
+    r_far = np.hypot(X[0, -1] - 0.5, X[1, -1])
+    np.testing.assert_allclose(r_far, 20.0, rtol=1e-12)
+    first = np.hypot(*(X[:, 1] - X[:, 0]))
+    assert 0.5e-3 < np.median(first) < 2e-3
+    # thickness of the NACA section at the max-thickness station ~ t
+    assert abs(2 * np.abs(X[1, 0]).max() - t) < 0.02 * t + 1e-3
+
+
+def test_plot3d_round_trip(tmp_path):
+    X = naca_ogrid(17, 9, 0.12)
+    f = tmp_path / "g.plot3d"
+    write_plot3d(f, X)
+    np.testing.assert_array_equal(read_plot3d(f), X)
+    head = f.read_text().splitlines()[:2]
+    assert head == ["1", "9 17"]
+
+
+def test_pitch_is_rigid_and_velocity_is_its_derivative():
+    X0 = naca_ogrid(33, 9, 0.12)
+    a, h = 0.03, 1e-6
+    X, V = pitch(X0, a, 1.0)
+    Xp, _ = pitch(X0, a + h, 1.0)
+    Xm, _ = pitch(X0, a - h, 1.0)
+    np.testing.assert_allclose(V, (Xp - Xm) / (2 * h), atol=1e-8)
+    np.testing.assert_allclose(_areas(X), _areas(X0), rtol=1e-12)
+    np.testing.assert_allclose(rotation_gain(X), V)
+
+
+# ------------------------------------------------------------------ the C ABI
+def _declared_symbols():
+    txt = (ROOT / "include" / "pea.h").read_text()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(pea_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    from pyeulerale_amd import _lib
+    lib = _lib.load()
+    syms = _declared_symbols()
+    assert len(syms) >= 20
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+    # the Python binding declares a signature for every exported entry point
+    assert set(syms) <= set(_lib._SIGS), set(syms) - set(_lib._SIGS)
+    assert lib.pea_version() >= 1
+
+
+def test_library_rejects_bad_dimensions_without_gpu_work():
+    import ctypes
+    from pyeulerale_amd import _lib
+    lib = _lib.load()
+    p = _lib.PeaParams(1.25, 1.0, 1 / 32, 2.0)
+    rc = lib.pea_compute_odes(ctypes.byref(p), 0.5, 3, 8, None, None, None, None, None)
+    assert rc == -1 and b"M >= 4" in lib.pea_last_error()
+    rc = lib.pea_compute_odes(ctypes.byref(p), 0.5, 8, 4, None, None, None, None, None)
+    assert rc == -1
+    rc = lib.pea_compute_odes(ctypes.byref(p), 0.5, 8, 8, None, None, None, None, None)
+    assert rc == -2
+    with pytest.raises(_lib.PeaError):
+        _lib.call("pea_jvp_odes_multi", ctypes.byref(p), 0.5, 8, 8, 3, *([None] * 9))
+
+
+def test_vjp_workspace_size_is_positive():
+    from pyeulerale_amd import _lib
+    assert _lib.load().pea_vjp_workspace_bytes(128, 256) >= 16
+    assert _lib.load().pea_vjp_workspace_bytes(3, 256) == 0
+
+
+def test_python_layer_checks_shapes_before_launch():
+    import torch
+    from pyeulerale_amd import kernels as K
+    g = K.Grid(8, 16)
+    with pytest.raises(RuntimeError):
+        K.Grid(3, 16)
+    bad = torch.zeros((4, 8, 16), dtype=torch.float64)   # reference shape, not storage
+    with pytest.raises(RuntimeError):
+        K.compute_odes(K.Params(), 0.5, g, torch.zeros(g.vert_shape, dtype=torch.float64),
+                       torch.zeros(g.vert_shape, dtype=torch.float64), bad)
