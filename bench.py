@@ -102,6 +102,7 @@ def products(torch, K, A, Rv, thickness, mach, aoa, alpha, alpha_dot, far, reps)
     ws = K._workspace(g, Ud.device)
     gU = torch.empty_like(dU)
     JU, _, _, _ = K.cs_linearize_odes(p, mach, g, Xd, Vd, Ud, want=("states",))
+    tile = K.autotune_vjp(p, mach, g, Xd, Vd, Ud)
     t = {}
     t["jvp"] = graph_time_ms(torch, lambda: K.jvp_odes(p, mach, g, Xd, Vd, Ud, dU=dU, dR=out),
                              reps)
@@ -117,14 +118,16 @@ def products(torch, K, A, Rv, thickness, mach, aoa, alpha, alpha_dot, far, reps)
     t["residual"] = graph_time_ms(torch, lambda: K.compute_odes(p, mach, g, Xd, Vd, Ud), reps)
     # accuracy of the hand-written products against complex step (h = 1e-12)
     jvp = K.jvp_odes(p, mach, g, Xd, Vd, Ud, dU=dU)
-    csd = K.cs_jvp_odes(p, mach, g, Xd, Vd, Ud, dU=dU)
+    csd = K.apply_states_fwd(g, JU, dU)
     vjp = K.vjp_odes(p, mach, g, Xd, Vd, Ud, w)["states"]
     asm = K.apply_states_rev(g, JU, w)
     C, P = M * N, (M + 1) * (N + 1)
     res = {
         "grid": f"{A}x{Rv}", "cells": C,
         "ms": {k: round(v, 5) for k, v in t.items()},
-        "rel_err_jvp_vs_cs": rel(jvp, csd), "rel_err_vjp_vs_cs_blocks": rel(vjp, asm),
+        "rel_err_jvp_vs_cs_blocks": rel(jvp, csd), "rel_err_vjp_vs_cs_blocks": rel(vjp, asm),
+        "rel_err_jvp_vs_cs_dir": rel(jvp, K.cs_jvp_odes(p, mach, g, Xd, Vd, Ud, dU=dU)),
+        "vjp_tile": tile,
         "speedup_vjp_vs_cs_dir": t["cs_dir"] / t["vjp"],
         "speedup_jvp_vs_cs_dir": t["cs_dir"] / t["jvp"],
         "speedup_vjp_vs_cs_asm_rev": t["cs_asm_apply_rev"] / t["vjp"],

@@ -44,10 +44,12 @@ enum pea_status {
 const char *pea_last_error(void);
 int pea_version(void);
 
-/* Adjoint tile-shape control for the tuner: pea_num_tiles() shapes exist; tile -1 restores the
- * built-in choice. */
+/* Tile shapes of the adjoint kernels (no counterpart in the reference: the tuner's control).
+ * pea_num_tiles() shapes exist; pea_set_tile registers the shape used for grids of M x N cells
+ * (M = N = 0: for every grid; tile = -1 clears); pea_get_tile reports the current choice. */
 int pea_num_tiles(void);
-int pea_set_tile_override(int tile);
+int pea_set_tile(int M, int N, int tile);
+int pea_get_tile(int M, int N);
 
 /* ------------------------------------------------------------------ primal */
 

@@ -28,7 +28,8 @@ class PeaParams(_c.Structure):
 _SIGS = {
     "pea_last_error": [],
     "pea_version": [],
-    "pea_set_tile_override": [_I],
+    "pea_set_tile": [_I, _I, _I],
+    "pea_get_tile": [_I, _I],
     "pea_num_tiles": [],
     "pea_set_free_stream_state": [_P, _D, _I, _I, _P, _P],
     "pea_compute_odes": [_P, _D, _I, _I, _P, _P, _P, _P, _P],

@@ -218,7 +218,7 @@ PEA_HD void cell_adj_init(const pea_params& P, const T& mach, const Src& src, co
                           const T (*x)[2], const T (*xw)[2], int flag, const T (*w)[L],
                           bool want_area, CellAdj<T, L>& ca) {
     cell_geometry(x, xw, ca.g);
-    if (flag == 1 || flag == 5) ghost_state(P, mach, uc, ca.g, flag, ca.gh);
+    if (flag == 1 || flag == 5) ghost_state<T, T, T>(P, mach, uc, ca.g, flag, ca.gh);
     const T ia = 1.0 / ca.g.area;
 #pragma unroll
     for (int k = 0; k < L; ++k) {

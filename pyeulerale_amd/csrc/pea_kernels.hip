@@ -7,9 +7,11 @@
 // neighbouring doubles in every plane; stencil reads of the angular neighbours hit L1/L2.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "pea_adjoint.hpp"
 
@@ -136,6 +138,46 @@ struct LdDual {
     }
 };
 
+// Tangents on the states only: geometry and Mach stay plain FP64.
+template <int K>
+struct LdDualU {
+    const double *X, *V, *U;
+    const double* dU;
+    double mach;
+    Grid g;
+    PEA_HD void state(int c, Dual<K>* u) const {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int e = v * g.C + c;
+            u[v].v = U[e];
+#pragma unroll
+            for (int j = 0; j < K; ++j) u[v].d[j] = dU[2 * ((long)(j >> 1) * 4 * g.C + e) + (j & 1)];
+        }
+    }
+    PEA_HD void vertex(int p, double* x) const { x[0] = X[p]; x[1] = X[g.P + p]; }
+    PEA_HD void velocity(int p, double* w) const { w[0] = V[p]; w[1] = V[g.P + p]; }
+    PEA_HD double machv() const { return mach; }
+};
+
+// Complex step on the states only (geometry and Mach real).
+struct LdCxU {
+    const double *X, *V, *U;
+    const double* dU;
+    double mach, h;
+    int part;
+    Grid g;
+    PEA_HD void state(int c, Cx* u) const {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int e = v * g.C + c;
+            u[v] = Cx{U[e], h * dU[2 * e + part]};
+        }
+    }
+    PEA_HD void vertex(int p, double* x) const { x[0] = X[p]; x[1] = X[g.P + p]; }
+    PEA_HD void velocity(int p, double* w) const { w[0] = V[p]; w[1] = V[g.P + p]; }
+    PEA_HD double machv() const { return mach; }
+};
+
 // Complex step along part `part` (0 real, 1 imaginary) of complex directions:
 // value + i h * part(direction).
 struct LdCx {
@@ -195,16 +237,17 @@ k_free_stream(pea_params P, double mach, int C, double* __restrict__ U) {
     for (int v = 0; v < 4; ++v) U[v * C + c] = u[v];
 }
 
-// One residual evaluation per thread.
-template <class T, class Ld, class Store>
+// One residual evaluation per thread; T state scalar, G geometry scalar.
+template <class T, class G, class Ld, class Store>
 __device__ __forceinline__ void residual_cell(const pea_params& P, const Ld& ld, int c,
                                               const Store& store) {
     const int n = c / ld.g.M, m = c - n * ld.g.M;
-    T x[4][2], w[4][2], uc[4];
-    load_cell_geometry<T>(ld, m, n, x, w);
+    G x[4][2], w[4][2];
+    T uc[4];
+    load_cell_geometry<G>(ld, m, n, x, w);
     ld.state(c, uc);
-    CellGeom<T> geo;
-    cell_geometry<T>(x, w, geo);
+    CellGeom<G> geo;
+    cell_geometry<G>(x, w, geo);
     T R[4];
     cell_ode_src(P, ld.machv(), CellSrc<Ld>{ld, m, n}, uc, geo, boundary_flag(ld.g.M, m), R);
     store(c, R);
@@ -227,7 +270,7 @@ k_odes(pea_params P, LdReal ld, double* __restrict__ R) {
     const long off = (long)blockIdx.y * 4 * ld.g.C;
     LdReal l = ld;
     l.U = ld.U + off;
-    residual_cell<double>(P, l, c, StoreReal{R + off, ld.g.C});
+    residual_cell<double, double>(P, l, c, StoreReal{R + off, ld.g.C});
 }
 
 // ------------------------------------------------------------------ tangent (JVP)
@@ -246,12 +289,13 @@ struct StoreDual {
     }
 };
 
-template <int K>
+template <int K, class Ld>
 __global__ void __launch_bounds__(256)
-k_jvp(pea_params P, LdDual<K> ld, double* __restrict__ dR, double* __restrict__ R) {
+k_jvp(pea_params P, Ld ld, double* __restrict__ dR, double* __restrict__ R) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ld.g.C) return;
-    residual_cell<Dual<K>>(P, ld, c, StoreDual<K>{dR, R, ld.g.C});
+    using G = decltype(ld.machv());
+    residual_cell<Dual<K>, G>(P, ld, c, StoreDual<K>{dR, R, ld.g.C});
 }
 
 // ------------------------------------------------------------------ complex step
@@ -265,16 +309,18 @@ struct StoreCs {
     }
 };
 
+template <class Ld>
 __global__ void __launch_bounds__(256)
-k_cs_jvp(pea_params P, LdCx ld, double* __restrict__ dR) {
+k_cs_jvp(pea_params P, Ld ld, double* __restrict__ dR) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ld.g.C) return;
-    LdCx l0 = ld;
+    using G = decltype(ld.machv());
+    Ld l0 = ld;
     l0.part = 0;
-    residual_cell<Cx>(P, l0, c, StoreCs{dR, ld.h, ld.g.C, 0});
-    LdCx l1 = ld;
+    residual_cell<Cx, G>(P, l0, c, StoreCs{dR, ld.h, ld.g.C, 0});
+    Ld l1 = ld;
     l1.part = 1;
-    residual_cell<Cx>(P, l1, c, StoreCs{dR, ld.h, ld.g.C, 1});
+    residual_cell<Cx, G>(P, l1, c, StoreCs{dR, ld.h, ld.g.C, 1});
 }
 
 // Complex-step block assembly (the reference's linearize): one thread per cell, one complex
@@ -904,38 +950,58 @@ int cell_block(long C) { return C >= 256L * 1024 ? 256 : 64; }
 struct TileShape {
     int tm, tn;
 };
-// Tile shapes the adjoint kernels are instantiated for (tn + 4 <= N is required so a box never
-// wraps onto itself).
-constexpr TileShape kTiles[] = {{28, 12}, {12, 12}, {28, 4}, {12, 1}};
-constexpr int kNumTiles = 4;
+// Tile shapes the adjoint kernels are instantiated for (tn + 4 <= N is required so that a box
+// never wraps onto itself).  The tangent-over-adjoint kernels (3x wider accumulators) use only
+// the shapes of at most 144 cells.
+constexpr TileShape kTiles[] = {{28, 28}, {60, 12}, {28, 12}, {12, 12}, {28, 4}, {12, 1}};
+constexpr int kNumTiles = 6;
 
-int g_tile_override = -1;
+PEA_HD constexpr int tile_rounds(int i) {
+    return ((kTiles[i].tm + 4) * (kTiles[i].tn + 4) + 255) / 256;
+}
 
-// `small`: only tiles of at most 144 cells (the tangent-over-adjoint accumulators are 3x wider)
-int pick_tile(const Grid& g, bool small) {
-    if (g_tile_override >= 0 && g_tile_override < kNumTiles &&
-        kTiles[g_tile_override].tn + 4 <= g.N &&
-        (!small || kTiles[g_tile_override].tm * kTiles[g_tile_override].tn <= 144))
-        return g_tile_override;
-    // prefer the largest tile that still yields >= 512 blocks; else the one with most blocks
-    int best = -1;
-    long best_blocks = -1;
-    for (int i = 0; i < kNumTiles; ++i) {
-        if (kTiles[i].tn + 4 > g.N || (small && kTiles[i].tm * kTiles[i].tn > 144)) continue;
-        const long nb = (long)((g.M + kTiles[i].tm - 1) / kTiles[i].tm) *
-                        ((g.N + kTiles[i].tn - 1) / kTiles[i].tn);
-        if (nb >= 512) return i;
-        if (nb > best_blocks) {
-            best_blocks = nb;
-            best = i;
-        }
-    }
-    return best;
+bool tile_fits(const Grid& g, int i, bool small) {
+    return i >= 0 && i < kNumTiles && kTiles[i].tn + 4 <= g.N &&
+           (!small || kTiles[i].tm * kTiles[i].tn <= 144);
 }
 
 long tile_blocks(const Grid& g, int i) {
     return (long)((g.M + kTiles[i].tm - 1) / kTiles[i].tm) *
            ((g.N + kTiles[i].tn - 1) / kTiles[i].tn);
+}
+
+// Per-grid choices registered by the tuner (M = N = 0: process-wide override).
+struct TileChoice {
+    int M, N, tile;
+};
+std::mutex g_tile_mutex;
+TileChoice g_tiles[64];
+int g_ntiles = 0;
+int g_tile_global = -1;
+
+// Default: minimise the estimated time max(rounds, blocks*rounds/512) -- 512 resident blocks
+// over 256 CUs -- then the total work.
+int pick_tile(const Grid& g, bool small) {
+    {
+        std::lock_guard<std::mutex> lk(g_tile_mutex);
+        if (tile_fits(g, g_tile_global, small)) return g_tile_global;
+        for (int i = 0; i < g_ntiles; ++i)
+            if (g_tiles[i].M == g.M && g_tiles[i].N == g.N && tile_fits(g, g_tiles[i].tile, small))
+                return g_tiles[i].tile;
+    }
+    int best = -1;
+    double best_t = 0.0, best_w = 0.0;
+    for (int i = 0; i < kNumTiles; ++i) {
+        if (!tile_fits(g, i, small)) continue;
+        const double work = (double)tile_blocks(g, i) * tile_rounds(i);
+        const double t = std::max((double)tile_rounds(i), work / 512.0);
+        if (best < 0 || t < best_t - 1e-9 || (t < best_t + 1e-9 && work < best_w)) {
+            best = i;
+            best_t = t;
+            best_w = work;
+        }
+    }
+    return best;
 }
 
 template <class T, int TM, int TN, class Ld>
@@ -964,15 +1030,18 @@ int launch_vjp(const pea_params& P, const Ld& ld, const double* w, double* gU, d
     int rc;
     if constexpr (small) {
         switch (tile) {
-        case 1: rc = launch_vjp_shape<T, 12, 12>(P, ld, w, gU, gX, gV, partial, st); break;
-        case 3: rc = launch_vjp_shape<T, 12, 1>(P, ld, w, gU, gX, gV, partial, st); break;
+        case 3: rc = launch_vjp_shape<T, 12, 12>(P, ld, w, gU, gX, gV, partial, st); break;
+        case 4: rc = launch_vjp_shape<T, 28, 4>(P, ld, w, gU, gX, gV, partial, st); break;
+        case 5: rc = launch_vjp_shape<T, 12, 1>(P, ld, w, gU, gX, gV, partial, st); break;
         default: return fail(PEA_ERR_ARG, "tile %d not available here", tile);
         }
     } else {
         switch (tile) {
-        case 0: rc = launch_vjp_shape<T, 28, 12>(P, ld, w, gU, gX, gV, partial, st); break;
-        case 1: rc = launch_vjp_shape<T, 12, 12>(P, ld, w, gU, gX, gV, partial, st); break;
-        case 2: rc = launch_vjp_shape<T, 28, 4>(P, ld, w, gU, gX, gV, partial, st); break;
+        case 0: rc = launch_vjp_shape<T, 28, 28>(P, ld, w, gU, gX, gV, partial, st); break;
+        case 1: rc = launch_vjp_shape<T, 60, 12>(P, ld, w, gU, gX, gV, partial, st); break;
+        case 2: rc = launch_vjp_shape<T, 28, 12>(P, ld, w, gU, gX, gV, partial, st); break;
+        case 3: rc = launch_vjp_shape<T, 12, 12>(P, ld, w, gU, gX, gV, partial, st); break;
+        case 4: rc = launch_vjp_shape<T, 28, 4>(P, ld, w, gU, gX, gV, partial, st); break;
         default: rc = launch_vjp_shape<T, 12, 1>(P, ld, w, gU, gX, gV, partial, st); break;
         }
     }
@@ -992,12 +1061,33 @@ extern "C" {
 const char* pea_last_error(void) { return g_err; }
 int pea_version(void) { return 1; }
 
-int pea_set_tile_override(int tile) {
+int pea_num_tiles(void) { return kNumTiles; }
+
+int pea_set_tile(int M, int N, int tile) {
     if (tile < -1 || tile >= kNumTiles) return fail(PEA_ERR_ARG, "tile %d out of range", tile);
-    g_tile_override = tile;
+    std::lock_guard<std::mutex> lk(g_tile_mutex);
+    if (M == 0 && N == 0) {
+        g_tile_global = tile;
+        return PEA_OK;
+    }
+    if (int rc = check_dims(M, N)) return rc;
+    if (tile >= 0 && !tile_fits(make_grid(M, N), tile, false))
+        return fail(PEA_ERR_ARG, "tile %d does not fit N=%d", tile, N);
+    for (int i = 0; i < g_ntiles; ++i)
+        if (g_tiles[i].M == M && g_tiles[i].N == N) {
+            g_tiles[i].tile = tile;
+            return PEA_OK;
+        }
+    if (tile < 0) return PEA_OK;
+    if (g_ntiles == 64) return fail(PEA_ERR_ARG, "tile registry full");
+    g_tiles[g_ntiles++] = TileChoice{M, N, tile};
     return PEA_OK;
 }
-int pea_num_tiles(void) { return kNumTiles; }
+
+int pea_get_tile(int M, int N) {
+    if (check_dims(M, N)) return -1;
+    return pick_tile(make_grid(M, N), false);
+}
 
 int pea_set_free_stream_state(const pea_params* prm, double mach, int M, int N, double* U,
                               void* stream) {
@@ -1050,8 +1140,12 @@ int pea_jvp_odes(const pea_params* prm, double mach, int M, int N, const double*
     if (!prm || !X || !V || !U || !dR) return fail(PEA_ERR_NULL, "null argument");
     const Grid g = make_grid(M, N);
     const int bs = cell_block(g.C);
-    k_jvp<2><<<blocks_for(g.C, bs), bs, 0, as_stream(stream)>>>(
-        *prm, LdDual<2>{X, V, U, dX, dV, dU, dmach, mach, g}, dR, R);
+    if (dU && !dX && !dV && !dmach)
+        k_jvp<2><<<blocks_for(g.C, bs), bs, 0, as_stream(stream)>>>(
+            *prm, LdDualU<2>{X, V, U, dU, mach, g}, dR, R);
+    else
+        k_jvp<2><<<blocks_for(g.C, bs), bs, 0, as_stream(stream)>>>(
+            *prm, LdDual<2>{X, V, U, dX, dV, dU, dmach, mach, g}, dR, R);
     PEA_LAUNCH_CHECK("k_jvp");
     return PEA_OK;
 }
@@ -1066,31 +1160,30 @@ int pea_jvp_odes_multi(const pea_params* prm, double mach, int M, int N, int k,
     const int bs = cell_block(g.C);
     const unsigned nb = blocks_for(g.C, bs);
     hipStream_t st = as_stream(stream);
-    switch (k) {
-    case 1:
-        k_jvp<2><<<nb, bs, 0, st>>>(*prm, LdDual<2>{X, V, U, dX, dV, dU, dmach, mach, g}, dR,
-                                    nullptr);
-        break;
-    case 2:
-        k_jvp<4><<<nb, bs, 0, st>>>(*prm, LdDual<4>{X, V, U, dX, dV, dU, dmach, mach, g}, dR,
-                                    nullptr);
-        break;
-    case 4:
-        k_jvp<8><<<nb, bs, 0, st>>>(*prm, LdDual<8>{X, V, U, dX, dV, dU, dmach, mach, g}, dR,
-                                    nullptr);
-        break;
-    case 8:
-        for (int h = 0; h < 2; ++h) {
-            const long su = 4L * 4 * 2 * g.C, sx = 4L * 2 * 2 * g.P;
-            k_jvp<8><<<nb, bs, 0, st>>>(
-                *prm,
-                LdDual<8>{X, V, U, dX ? dX + h * sx : nullptr, dV ? dV + h * sx : nullptr,
-                          dU ? dU + h * su : nullptr, dmach ? dmach + 8 * h : nullptr, mach, g},
-                dR + h * su, nullptr);
-        }
-        break;
-    default:
+    if (k != 1 && k != 2 && k != 4 && k != 8)
         return fail(PEA_ERR_ARG, "k must be 1, 2, 4 or 8 (got %d)", k);
+    const bool u_only = dU && !dX && !dV && !dmach;
+    // directions are processed two at a time (four tangent lanes per pass)
+    const long su = 2L * 4 * 2 * g.C, sx = 2L * 2 * 2 * g.P;
+    for (int h = 0; h < k; h += 2) {
+        const long o = h / 2;
+        double* out = dR + o * su;
+        if (k == 1) {
+            if (u_only)
+                k_jvp<2><<<nb, bs, 0, st>>>(*prm, LdDualU<2>{X, V, U, dU, mach, g}, out, nullptr);
+            else
+                k_jvp<2><<<nb, bs, 0, st>>>(
+                    *prm, LdDual<2>{X, V, U, dX, dV, dU, dmach, mach, g}, out, nullptr);
+        } else if (u_only) {
+            k_jvp<4><<<nb, bs, 0, st>>>(*prm, LdDualU<4>{X, V, U, dU + o * su, mach, g}, out,
+                                        nullptr);
+        } else {
+            k_jvp<4><<<nb, bs, 0, st>>>(
+                *prm,
+                LdDual<4>{X, V, U, dX ? dX + o * sx : nullptr, dV ? dV + o * sx : nullptr,
+                          dU ? dU + o * su : nullptr, dmach ? dmach + 4 * o : nullptr, mach, g},
+                out, nullptr);
+        }
     }
     PEA_LAUNCH_CHECK("k_jvp");
     return PEA_OK;
@@ -1111,7 +1204,7 @@ long long pea_vjp_workspace_bytes(int M, int N) {
     const Grid g = make_grid(M, N);
     long most = 1;
     for (int i = 0; i < kNumTiles; ++i)
-        if (kTiles[i].tn + 4 <= N && tile_blocks(g, i) > most) most = tile_blocks(g, i);
+        if (tile_fits(g, i, false) && tile_blocks(g, i) > most) most = tile_blocks(g, i);
     return (long long)most * 2 * (long long)sizeof(double);
 }
 
@@ -1179,8 +1272,12 @@ int pea_cs_jvp_odes(const pea_params* prm, double mach, int M, int N, const doub
     if (!(h > 0.0)) h = kCsStep;
     const Grid g = make_grid(M, N);
     const int bs = cell_block(g.C);
-    k_cs_jvp<<<blocks_for(g.C, bs), bs, 0, as_stream(stream)>>>(
-        *prm, LdCx{X, V, U, dX, dV, dU, dmach, mach, h, 0, g}, dR);
+    if (dU && !dX && !dV && !dmach)
+        k_cs_jvp<<<blocks_for(g.C, bs), bs, 0, as_stream(stream)>>>(
+            *prm, LdCxU{X, V, U, dU, mach, h, 0, g}, dR);
+    else
+        k_cs_jvp<<<blocks_for(g.C, bs), bs, 0, as_stream(stream)>>>(
+            *prm, LdCx{X, V, U, dX, dV, dU, dmach, mach, h, 0, g}, dR);
     PEA_LAUNCH_CHECK("k_cs_jvp");
     return PEA_OK;
 }

@@ -38,8 +38,9 @@ PEA_HD void edge_normal(const T* a, const T* b, T* n) {
 }
 
 // Physical flux through an edge (f90:159-171); q = n.j/rho is returned for reuse.
-template <class T>
-PEA_HD void physical_flux(const T* u, const T* n, T* f, T& q, T& p) {
+// T: state scalar, G: geometry scalar (double when the geometry carries no tangent).
+template <class T, class G>
+PEA_HD void physical_flux(const T* u, const G* n, T* f, T& q, T& p) {
     q = (n[0] * u[1] + n[1] * u[2]) / u[0];
     p = pressure(u);
     f[0] = u[0] * q;
@@ -50,16 +51,16 @@ PEA_HD void physical_flux(const T* u, const T* n, T* f, T& q, T& p) {
 
 // JST numerical flux with the ALE term (f90:189-220).  With diss == false the face is central
 // only (the reference's absent ll/rr optional arguments).
-template <class T>
+template <class T, class G>
 PEA_HD void face_flux(const pea_params& P, const T* ll, const T* l, const T* r, const T* rr,
-                      const T* n, const T* gv, bool diss, T* f) {
+                      const G* n, const G* gv, bool diss, T* f) {
     T fl[4], fr[4], ql, pl, qr, pr;
     physical_flux(l, n, fl, ql, pl);
     physical_flux(r, n, fr, qr, pr);
 #pragma unroll
     for (int v = 0; v < 4; ++v) f[v] = (fl[v] + fr[v]) / 2.0;
     if (diss) {
-        const T sn = ssqrt(n[0] * n[0] + n[1] * n[1]);
+        const G sn = ssqrt(n[0] * n[0] + n[1] * n[1]);
         const T lam_l = sabs(ql) + ssqrt(kGamma * pl / l[0]) * sn;   // f90:146-147
         const T lam_r = sabs(qr) + ssqrt(kGamma * pr / r[0]) * sn;
         const T lam = maxb(lam_l, lam_r);
@@ -72,7 +73,7 @@ PEA_HD void face_flux(const pea_params& P, const T* ll, const T* l, const T* r, 
         for (int v = 0; v < 4; ++v)
             f[v] = f[v] - (e2 * (r[v] - l[v]) - e4 * (rr[v] - 3.0 * r[v] + 3.0 * l[v] - ll[v]));
     }
-    const T vn = gv[0] * n[0] + gv[1] * n[1];
+    const G vn = gv[0] * n[0] + gv[1] * n[1];
 #pragma unroll
     for (int v = 0; v < 4; ++v) f[v] = f[v] - (l[v] + r[v]) / 2.0 * vn;
 }
@@ -89,10 +90,10 @@ PEA_HD void free_stream(const pea_params& P, const T& mach, T* u) {
 }
 
 // Wall ghost state (f90:231-240).
-template <class T>
-PEA_HD void wall_ghost(const T* u, const T* n, const T* gv, T* g) {
+template <class T, class G>
+PEA_HD void wall_ghost(const T* u, const G* n, const G* gv, T* g) {
     const T s = n[0] * (u[1] - gv[0] * u[0]) + n[1] * (u[2] - gv[1] * u[0]);
-    const T nn = n[0] * n[0] + n[1] * n[1];
+    const G nn = n[0] * n[0] + n[1] * n[1];
     g[0] = u[0];
     g[1] = u[1] - 2.0 * n[0] * s / nn;
     g[2] = u[2] - 2.0 * n[1] * s / nn;
@@ -164,13 +165,13 @@ PEA_HD FaceArgs face_args(int flag) {
 }
 
 // Ghost state of a boundary cell (wall: flag 1, far field: flag 5).
-template <class T>
-PEA_HD void ghost_state(const pea_params& P, const T& mach, const T* uc, const CellGeom<T>& g,
+template <class T, class G, class A>
+PEA_HD void ghost_state(const pea_params& P, const A& mach, const T* uc, const CellGeom<G>& g,
                         int flag, T* gh) {
     if (flag == 1) {
         wall_ghost(uc, g.n[FI_], g.gv[FI_], gh);
     } else {
-        T inf[4];
+        A inf[4];
         free_stream(P, mach, inf);
 #pragma unroll
         for (int v = 0; v < 4; ++v) gh[v] = 2.0 * inf[v] - uc[v];   // f90:269-274
@@ -180,14 +181,14 @@ PEA_HD void ghost_state(const pea_params& P, const T& mach, const T* uc, const C
 // Residual of one cell (f90:370-436): R = -(f_b + f_f + f_i + f_o) / area.
 // `src.get(slot, u)` supplies the state of stencil slot `slot` (Slot order); states are fetched
 // face by face so only one face's four arguments are live at a time.  `uc` is the centre state.
-template <class T, class Src>
-PEA_HD void cell_ode_src(const pea_params& P, const T& mach, const Src& src, const T* uc,
-                         const CellGeom<T>& g, int flag, T* R) {
+template <class T, class G, class A, class Src>
+PEA_HD void cell_ode_src(const pea_params& P, const A& mach, const Src& src, const T* uc,
+                         const CellGeom<G>& g, int flag, T* R) {
     T gh[4];
     if (flag == 1) {
         wall_ghost(uc, g.n[FI_], g.gv[FI_], gh);
     } else if (flag == 5) {
-        T inf[4];
+        A inf[4];
         free_stream(P, mach, inf);
 #pragma unroll
         for (int v = 0; v < 4; ++v) gh[v] = 2.0 * inf[v] - uc[v];   // f90:269-274

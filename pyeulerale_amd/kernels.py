@@ -204,6 +204,37 @@ def _workspace(g: Grid, device) -> torch.Tensor:
     return torch.empty((max(nbytes, 16) + 7) // 8, dtype=F64, device=device)
 
 
+_TUNED: dict = {}
+
+
+def autotune_vjp(p: Params, mach: float, g: Grid, X, V, U, reps: int = 5) -> int:
+    """Times every adjoint tile shape that fits this grid and registers the fastest
+    (cached per grid size).  Returns the chosen tile index."""
+    key = (g.M, g.N)
+    if key in _TUNED:
+        return _TUNED[key]
+    lib = _lib.load()
+    gen = torch.Generator(device=U.device).manual_seed(0)
+    w = torch.randn(g.cell_shape, dtype=C128, device=U.device, generator=gen)
+    out = {"states": torch.empty(g.cell_shape, dtype=C128, device=U.device)}
+    times = {}
+    for t in range(lib.pea_num_tiles()):
+        if lib.pea_set_tile(g.M, g.N, t) != 0:
+            continue
+        vjp_odes(p, mach, g, X, V, U, w, out=out)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            vjp_odes(p, mach, g, X, V, U, w, out=out)
+        b.record()
+        b.synchronize()
+        times[t] = a.elapsed_time(b) / reps
+    best = min(times, key=times.get)
+    _lib.call("pea_set_tile", g.M, g.N, best)
+    _TUNED[key] = best
+    return best
+
+
 def vjp_odes(p: Params, mach: float, g: Grid, X, V, U, w, want=("states",), out=None):
     """Transposed products with complex seed w (no conjugation).  Returns a dict with the
     requested subset of 'states', 'vertices', 'velocities', 'mach'."""
