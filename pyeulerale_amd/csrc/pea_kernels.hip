@@ -14,6 +14,7 @@
 #include <mutex>
 
 #include "pea_adjoint.hpp"
+#include "pea_face.hpp"
 
 using namespace pea;
 
@@ -297,6 +298,231 @@ k_jvp(pea_params P, Ld ld, double* __restrict__ dR, double* __restrict__ R) {
     using G = decltype(ld.machv());
     residual_cell<Dual<K>, G>(P, ld, c, StoreDual<K>{dR, R, ld.g.C});
 }
+
+// ------------------------------------------------------------------ face-based forward
+// Residual (T = double) or residual + tangent (T = Dual<K>) of a TM x TN tile of cells, one
+// thread per cell.  Every face touching the tile is evaluated once (face_flux2): a thread owns
+// its cell's f and o faces, hands the opposite side's flux to its neighbour through LDS, and
+// the tile's first column / first row also evaluate the faces shared with the neighbouring
+// tiles.  Per-cell pressures and sound speeds of the tile grown by two cells are computed once
+// into LDS; the wall and far-field ghosts sit in the grown rows m = -1 and m = M.  Each cell
+// then sums its four face fluxes in the reference's order (b, f, i, o; f90:395-433).
+// blockIdx.z selects one of several states on the same grid (consecutive 4*C arrays).
+template <class T, class G, int TM, int TN, class Ld, class Store>
+__global__ void __launch_bounds__(TM * TN)
+k_face_fwd(pea_params P, Ld ld, Store store) {
+    constexpr int NT = TM * TN, BM = TM + 4, BN = TN + 4, BOX = BM * BN;
+    __shared__ T sP[BOX], sC[BOX];
+    __shared__ T sG[2][BN][4];
+    __shared__ T sFb[NT][4], sFi[NT][4];
+    const Grid g = ld.g;
+    Ld l = ld;
+    l.U = ld.U + (long)blockIdx.z * 4 * g.C;
+    const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
+    const int tid = threadIdx.x;
+    const int tm = tid % TM, tn = tid / TM;
+    const int m = m0 + tm, n = n0 + tn;
+    const bool live = m < g.M && n < g.N;
+    const auto mach = l.machv();
+
+    auto vtx = [&](int mm, int nn, G* x) { l.vertex(nn * (g.M + 1) + mm, x); };
+    auto vel = [&](int mm, int nn, G* w) { l.velocity(nn * (g.M + 1) + mm, w); };
+    // normal of the edge from vertex (ma, na) to vertex (mb, nb)
+    auto nrm = [&](int ma, int na, int mb, int nb, G* nr) {
+        G xa[2], xb[2];
+        vtx(ma, na, xa);
+        vtx(mb, nb, xb);
+        edge_normal(xa, xb, nr);
+    };
+    // ... and the mean velocity of its two vertices
+    auto edge = [&](int ma, int na, int mb, int nb, G* nr, G* gv) {
+        G xa[2], xb[2], va[2], vb[2];
+        vtx(ma, na, xa);
+        vtx(mb, nb, xb);
+        vel(ma, na, va);
+        vel(mb, nb, vb);
+        edge_normal(xa, xb, nr);
+        gv[0] = (va[0] + vb[0]) / 2.0;
+        gv[1] = (va[1] + vb[1]) / 2.0;
+    };
+
+    // 1. pressures, sound speeds and ghosts over the grown tile
+    for (int i = tid; i < BOX; i += NT) {
+        const int bm = i % BM, bn = i / BM;
+        const int cm = m0 - 2 + bm;
+        const int cn = ((n0 - 2 + bn) % g.N + g.N) % g.N;
+        if (cm >= 0 && cm < g.M) {
+            T u[4];
+            l.state(cn * g.M + cm, u);
+            cell_pc(u, sP[i], sC[i]);
+        } else if (cm == -1 || cm == g.M) {
+            const int mc = cm < 0 ? 0 : g.M - 1;
+            T u[4], gh[4];
+            l.state(cn * g.M + mc, u);
+            if (cm < 0) {
+                G nr[2], gv[2];
+                edge(0, cn, 0, cn + 1, nr, gv);       // wall face bi -> fi
+                wall_ghost(u, nr, gv, gh);
+            } else {
+                decltype(l.machv()) inf[4];
+                free_stream(P, mach, inf);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) gh[v] = 2.0 * inf[v] - u[v];
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sG[cm < 0 ? 0 : 1][bn][v] = gh[v];
+            sP[i] = pressure(gh);
+        }
+    }
+    __syncthreads();
+
+    auto bidx = [&](int mm, int nb) { return nb * BM + (mm - m0 + 2); };   // nb: box column
+    T Ff[4], Fo[4], Fb[4], Fi[4];
+    const int bn = tn + 2;
+    if (live) {
+        T uc[4];
+        const int c = n * g.M + m;
+        l.state(c, uc);
+        // ---- angular face n | n+1 (this cell's f face)
+        {
+            T Rs[4], LLs[4], RRs[4], fR[4];
+            l.state(wrapn(n + 1, g.N) * g.M + m, Rs);
+            l.state(wrapn(n - 1, g.N) * g.M + m, LLs);
+            l.state(wrapn(n + 2, g.N) * g.M + m, RRs);
+            G nr[2], nr2[2], gv[2];
+            edge(m, n + 1, m + 1, n + 1, nr, gv);     // fi -> fo
+            nrm(m + 1, n + 1, m, n + 1, nr2);         // R's bo -> bi
+            face_flux2(P, LLs, uc, Rs, RRs, sP[bidx(m, bn - 1)], sP[bidx(m, bn)],
+                       sP[bidx(m, bn + 1)], sP[bidx(m, bn + 2)], sC[bidx(m, bn)],
+                       sC[bidx(m, bn + 1)], nr, nr2, gv, true, true, Ff, fR);
+            if (tn + 1 < TN && n + 1 < g.N)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) sFb[tid + TM][v] = fR[v];
+        }
+        // ---- radial face m | m+1 (this cell's o face), or the far-field face
+        if (m < g.M - 1) {
+            T Rs[4], LLs[4], RRs[4], fR[4];
+            l.state(c + 1, Rs);
+            if (m == 0) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) LLs[v] = sG[0][bn][v];
+            } else {
+                l.state(c - 1, LLs);
+            }
+            if (m + 1 == g.M - 1) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) RRs[v] = sG[1][bn][v];
+            } else {
+                l.state(c + 2, RRs);
+            }
+            G nr[2], nr2[2], gv[2];
+            edge(m + 1, n + 1, m + 1, n, nr, gv);     // fo -> bo
+            nrm(m + 1, n, m + 1, n + 1, nr2);         // R's bi -> fi
+            face_flux2(P, LLs, uc, Rs, RRs, sP[bidx(m - 1, bn)], sP[bidx(m, bn)],
+                       sP[bidx(m + 1, bn)], sP[bidx(m + 2, bn)], sC[bidx(m, bn)],
+                       sC[bidx(m + 1, bn)], nr, nr2, gv, m != g.M - 2, m + 1 != 1, Fo, fR);
+            if (tm + 1 < TM)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) sFi[tid + 1][v] = fR[v];
+        } else {
+            T gh[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) gh[v] = sG[1][bn][v];
+            G nr[2], gv[2];
+            edge(m + 1, n + 1, m + 1, n, nr, gv);
+            face_flux(P, uc, uc, gh, gh, nr, gv, false, Fo);
+        }
+        // ---- faces shared with the neighbouring tiles
+        if (tn == 0) {   // angular face n-1 | n; normal L -> R is minus this cell's b normal
+            T Ls[4], LLs[4], RRs[4], fL[4];
+            l.state(wrapn(n - 1, g.N) * g.M + m, Ls);
+            l.state(wrapn(n - 2, g.N) * g.M + m, LLs);
+            l.state(wrapn(n + 1, g.N) * g.M + m, RRs);
+            G nr[2], nr2[2], gv[2];
+            edge(m, n, m + 1, n, nr, gv);             // bi -> bo
+            nrm(m + 1, n, m, n, nr2);                 // this cell's bo -> bi
+            face_flux2(P, LLs, Ls, uc, RRs, sP[bidx(m, bn - 2)], sP[bidx(m, bn - 1)],
+                       sP[bidx(m, bn)], sP[bidx(m, bn + 1)], sC[bidx(m, bn - 1)],
+                       sC[bidx(m, bn)], nr, nr2, gv, false, true, fL, Fb);
+        }
+        if (tm == 0) {
+            if (m == 0) {   // wall face: central against the wall ghost
+                T gh[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) gh[v] = sG[0][bn][v];
+                G nr[2], gv[2];
+                edge(0, n, 0, n + 1, nr, gv);          // bi -> fi
+                face_flux(P, uc, uc, gh, gh, nr, gv, false, Fi);
+            } else {        // radial face m-1 | m; normal L -> R is minus this cell's i normal
+                T Ls[4], LLs[4], RRs[4], fL[4];
+                l.state(c - 1, Ls);
+                if (m - 1 == 0) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) LLs[v] = sG[0][bn][v];
+                } else {
+                    l.state(c - 2, LLs);
+                }
+                if (m == g.M - 1) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) RRs[v] = sG[1][bn][v];
+                } else {
+                    l.state(c + 1, RRs);
+                }
+                G nr[2], nr2[2], gv[2];
+                edge(m, n + 1, m, n, nr, gv);          // fo -> bo of L
+                nrm(m, n, m, n + 1, nr2);              // this cell's bi -> fi
+                face_flux2(P, LLs, Ls, uc, RRs, sP[bidx(m - 2, bn)], sP[bidx(m - 1, bn)],
+                           sP[bidx(m, bn)], sP[bidx(m + 1, bn)], sC[bidx(m - 1, bn)],
+                           sC[bidx(m, bn)], nr, nr2, gv, false, m != 1, fL, Fi);
+            }
+        }
+    }
+    __syncthreads();
+    if (live) {
+        if (tn > 0)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Fb[v] = sFb[tid][v];
+        if (tm > 0)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Fi[v] = sFi[tid][v];
+        G x[4][2];
+        vtx(m, n, x[BI]);
+        vtx(m, n + 1, x[FI]);
+        vtx(m + 1, n + 1, x[FO]);
+        vtx(m + 1, n, x[BO]);
+        const G d1x = x[FO][0] - x[BI][0], d1z = x[FO][1] - x[BI][1];
+        const G d2x = x[BO][0] - x[FI][0], d2z = x[BO][1] - x[FI][1];
+        const G area = sabs(d1x * d2z - d1z * d2x) / 2.0;
+        T R[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) R[v] = -((((Fb[v] + Ff[v]) + Fi[v]) + Fo[v]) / area);
+        store.put(blockIdx.z, n * g.M + m, R);
+    }
+}
+
+struct StoreRealB {
+    double* R;
+    int C;
+    PEA_HD void put(int b, int c, const double* r) const {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) R[(long)b * 4 * C + (long)v * C + c] = r[v];
+    }
+};
+
+template <int K>
+struct StoreDualB {
+    double *dR, *R;
+    int C;
+    PEA_HD void put(int, int c, const Dual<K>* r) const {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const long e = (long)v * C + c;
+            if (R) R[e] = r[v].v;
+#pragma unroll
+            for (int j = 0; j < K; ++j) dR[2 * ((long)(j >> 1) * 4 * C + e) + (j & 1)] = r[v].d[j];
+        }
+    }
+};
 
 // ------------------------------------------------------------------ complex step
 struct StoreCs {
@@ -947,6 +1173,13 @@ unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 // cell kernels: smaller blocks when the grid is small so that all CUs get work
 int cell_block(long C) { return C >= 256L * 1024 ? 256 : 64; }
 
+// tiles of the face-based forward kernels (m fastest within a tile)
+constexpr int kFwdTM = 32, kFwdTN = 8;
+constexpr int kJvpTM = 32, kJvpTN = 4;
+dim3 fwd_grid(const Grid& g, int nstates, int tm = kFwdTM, int tn = kFwdTN) {
+    return dim3((g.M + tm - 1) / tm, (g.N + tn - 1) / tn, nstates);
+}
+
 struct TileShape {
     int tm, tn;
 };
@@ -1104,9 +1337,10 @@ int pea_compute_odes(const pea_params* prm, double mach, int M, int N, const dou
     if (int rc = check_dims(M, N)) return rc;
     if (!prm || !X || !V || !U || !R) return fail(PEA_ERR_NULL, "null argument");
     const Grid g = make_grid(M, N);
-    const int bs = cell_block(g.C);
-    k_odes<<<blocks_for(g.C, bs), bs, 0, as_stream(stream)>>>(*prm, LdReal{X, V, U, mach, g}, R);
-    PEA_LAUNCH_CHECK("k_odes");
+    k_face_fwd<double, double, kFwdTM, kFwdTN><<<fwd_grid(g, 1), kFwdTM * kFwdTN, 0,
+                                                 as_stream(stream)>>>(
+        *prm, LdReal{X, V, U, mach, g}, StoreRealB{R, g.C});
+    PEA_LAUNCH_CHECK("k_face_fwd");
     return PEA_OK;
 }
 
@@ -1117,10 +1351,10 @@ int pea_compute_odes_batch(const pea_params* prm, double mach, int M, int N, int
     if (!prm || !X || !V || !U || !R) return fail(PEA_ERR_NULL, "null argument");
     if (nstates < 1 || nstates > 65535) return fail(PEA_ERR_ARG, "nstates %d", nstates);
     const Grid g = make_grid(M, N);
-    const int bs = cell_block((long)g.C * nstates);
-    k_odes<<<dim3(blocks_for(g.C, bs), nstates), bs, 0, as_stream(stream)>>>(
-        *prm, LdReal{X, V, U, mach, g}, R);
-    PEA_LAUNCH_CHECK("k_odes");
+    k_face_fwd<double, double, kFwdTM, kFwdTN><<<fwd_grid(g, nstates), kFwdTM * kFwdTN, 0,
+                                                 as_stream(stream)>>>(
+        *prm, LdReal{X, V, U, mach, g}, StoreRealB{R, g.C});
+    PEA_LAUNCH_CHECK("k_face_fwd");
     return PEA_OK;
 }
 
@@ -1140,13 +1374,16 @@ int pea_jvp_odes(const pea_params* prm, double mach, int M, int N, const double*
     if (!prm || !X || !V || !U || !dR) return fail(PEA_ERR_NULL, "null argument");
     const Grid g = make_grid(M, N);
     const int bs = cell_block(g.C);
+    (void)bs;
     if (dU && !dX && !dV && !dmach)
-        k_jvp<2><<<blocks_for(g.C, bs), bs, 0, as_stream(stream)>>>(
-            *prm, LdDualU<2>{X, V, U, dU, mach, g}, dR, R);
+        k_face_fwd<Dual<2>, double, kJvpTM, kJvpTN><<<fwd_grid(g, 1, kJvpTM, kJvpTN),
+                                                      kJvpTM * kJvpTN, 0, as_stream(stream)>>>(
+            *prm, LdDualU<2>{X, V, U, dU, mach, g}, StoreDualB<2>{dR, R, g.C});
     else
-        k_jvp<2><<<blocks_for(g.C, bs), bs, 0, as_stream(stream)>>>(
-            *prm, LdDual<2>{X, V, U, dX, dV, dU, dmach, mach, g}, dR, R);
-    PEA_LAUNCH_CHECK("k_jvp");
+        k_face_fwd<Dual<2>, Dual<2>, kJvpTM, kJvpTN><<<fwd_grid(g, 1, kJvpTM, kJvpTN),
+                                                       kJvpTM * kJvpTN, 0, as_stream(stream)>>>(
+            *prm, LdDual<2>{X, V, U, dX, dV, dU, dmach, mach, g}, StoreDualB<2>{dR, R, g.C});
+    PEA_LAUNCH_CHECK("k_face_fwd");
     return PEA_OK;
 }
 
