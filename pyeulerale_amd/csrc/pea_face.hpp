@@ -107,6 +107,177 @@ PEA_HD void face_flux2(const pea_params& P, const T* LL, const T* L, const T* R,
     }
 }
 
+// ---------------------------------------------------------------------------- reverse
+// Reverse of side_dissipation with seed db on D: accumulates into the adjoints of the side's
+// ll, l, r, rr states (sb*), of its two wave integrals (lamb_l, lamb_r) and of the four
+// pressures entering its sensors (pb_ll, pb_l, pb_r, pb_rr).  Switch decisions are recomputed.
+template <class T, int L>
+PEA_HD void side_dissipation_rev(const pea_params& P, const T& lam_l, const T& lam_r,
+                                 const T& pll, const T& pl, const T& pr, const T& prr,
+                                 const T* ll, const T* l, const T* r, const T* rr,
+                                 const T (*db)[L], T (*sbll)[L], T (*sbl)[L], T (*sbr)[L],
+                                 T (*sbrr)[L], T* lamb_l, T* lamb_r, T* pbll, T* pbl, T* pbr,
+                                 T* pbrr) {
+    const bool take_l = maxb_takes_first(lam_l, lam_r);
+    const T lam = take_l ? lam_l : lam_r;
+    const T a1 = pll - 2.0 * pl + pr, b1 = pll + 2.0 * pl + pr;
+    const T a2 = pl - 2.0 * pr + prr, b2 = pl + 2.0 * pr + prr;
+    const T s1 = sabs(a1) / b1, s2 = sabs(a2) / b2;
+    const bool take_s1 = maxb_takes_first(s1, s2);
+    const T s = take_s1 ? s1 : s2;
+    const T e2 = P.k2 * s * lam;
+    const T t4 = P.k4 * lam - P.c4 * e2;
+    const bool zero4 = maxb_takes_first(lit<T>(0.0), t4);
+    const T e4 = zero4 ? lit<T>(0.0) : t4;
+    const double sg1 = sign1(a1), sg2 = sign1(a2);
+    const T ib1 = 1.0 / b1, ib2 = 1.0 / b2;
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+        T e2b = lit<T>(0.0), e4b = lit<T>(0.0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const T d = db[v][k];
+            e2b = e2b + d * (r[v] - l[v]);
+            e4b = e4b - d * (rr[v] - 3.0 * r[v] + 3.0 * l[v] - ll[v]);
+            const T t2 = d * e2, tq = d * e4;
+            sbr[v][k] = sbr[v][k] + t2 + 3.0 * tq;
+            sbl[v][k] = sbl[v][k] - t2 - 3.0 * tq;
+            sbrr[v][k] = sbrr[v][k] - tq;
+            sbll[v][k] = sbll[v][k] + tq;
+        }
+        const T t4b = zero4 ? lit<T>(0.0) : e4b;
+        T lamb = P.k4 * t4b;
+        e2b = e2b - P.c4 * t4b;
+        const T sbar = P.k2 * lam * e2b;
+        lamb = lamb + P.k2 * s * e2b;
+        const T s1b = take_s1 ? sbar : lit<T>(0.0);
+        const T s2b = take_s1 ? lit<T>(0.0) : sbar;
+        const T a1b = sg1 * s1b * ib1, b1b = -(s1b * s1 * ib1);
+        const T a2b = sg2 * s2b * ib2, b2b = -(s2b * s2 * ib2);
+        pbll[k] = pbll[k] + a1b + b1b;
+        pbl[k] = pbl[k] - 2.0 * a1b + 2.0 * b1b + a2b + b2b;
+        pbr[k] = pbr[k] + a1b + b1b - 2.0 * a2b + 2.0 * b2b;
+        pbrr[k] = pbrr[k] + a2b + b2b;
+        lamb_l[k] = lamb_l[k] + (take_l ? lamb : lit<T>(0.0));
+        lamb_r[k] = lamb_r[k] + (take_l ? lit<T>(0.0) : lamb);
+    }
+}
+
+// Adjoint contributions of one face to its four cells (index 0..3 = LL, L, R, RR).
+template <class T, int L>
+struct FaceAdj {
+    T u[4][4][L];   // direct state adjoints
+    T p[4][L];      // pressure adjoints
+    T c[2][L];      // sound-speed adjoints of L and R
+};
+
+template <class T, int L>
+PEA_HD void face_adj_zero(FaceAdj<T, L>& a) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int k = 0; k < L; ++k) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) a.u[j][v][k] = lit<T>(0.0);
+            a.p[j][k] = lit<T>(0.0);
+            if (j < 2) a.c[j][k] = lit<T>(0.0);
+        }
+}
+
+// Reverse of face_flux2 with seeds fbL on f_L and fbR on f_R (the faces' shares of the two
+// cells' residual seeds, -w/area).  Accumulates into `a`; pressures and sound speeds are
+// treated as inputs (their adjoints go to a.p / a.c).
+template <class T, class G, int L>
+PEA_HD void face_flux2_rev(const pea_params& P, const T* LL, const T* Lc, const T* R,
+                           const T* RR, const T& pLL, const T& pL, const T& pR, const T& pRR,
+                           const T& cL, const T& cR, const G* n, const G* nR, const G* gv,
+                           bool dissL, bool dissR, const T (*fbL)[L], const T (*fbR)[L],
+                           FaceAdj<T, L>& a) {
+    const T iL = 1.0 / Lc[0], iR = 1.0 / R[0];
+    const T qL = (n[0] * Lc[1] + n[1] * Lc[2]) / Lc[0];
+    const T qR = (n[0] * R[1] + n[1] * R[2]) / R[0];
+    const G vn = gv[0] * n[0] + gv[1] * n[1];
+    T qLb[L], qRb[L];
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+        qLb[k] = lit<T>(0.0);
+        qRb[k] = lit<T>(0.0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            // f_L = cen - D_L - ale,  f_R = -cen - D_R + ale
+            const T ceb = (fbL[v][k] - fbR[v][k]) / 2.0;    // seed on phi(L) and phi(R)
+            const T aleb = (fbR[v][k] - fbL[v][k]) * vn / 2.0;
+            a.u[1][v][k] = a.u[1][v][k] + aleb + ceb * qL;
+            a.u[2][v][k] = a.u[2][v][k] + aleb + ceb * qR;
+        }
+        const T fL0 = (fbL[0][k] - fbR[0][k]) / 2.0, fL1 = (fbL[1][k] - fbR[1][k]) / 2.0,
+                fL2 = (fbL[2][k] - fbR[2][k]) / 2.0, fL3 = (fbL[3][k] - fbR[3][k]) / 2.0;
+        qLb[k] = fL0 * Lc[0] + fL1 * Lc[1] + fL2 * Lc[2] + fL3 * (Lc[3] + pL);
+        qRb[k] = fL0 * R[0] + fL1 * R[1] + fL2 * R[2] + fL3 * (R[3] + pR);
+        const T pn = fL1 * n[0] + fL2 * n[1];
+        a.p[1][k] = a.p[1][k] + pn + fL3 * qL;
+        a.p[2][k] = a.p[2][k] + pn + fL3 * qR;
+    }
+    if (dissL || dissR) {
+        const G sn = ssqrt(n[0] * n[0] + n[1] * n[1]);
+        if (dissL) {
+            const double sL = sign1(qL), sR = sign1(qR);
+            const T lamL = sabs(qL) + cL * sn, lamR = sabs(qR) + cR * sn;
+            T db[4][L], lbl[L], lbr[L];
+#pragma unroll
+            for (int k = 0; k < L; ++k) {
+                lbl[k] = lbr[k] = lit<T>(0.0);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) db[v][k] = -fbL[v][k];
+            }
+            side_dissipation_rev<T, L>(P, lamL, lamR, pLL, pL, pR, pRR, LL, Lc, R, RR, db,
+                                       a.u[0], a.u[1], a.u[2], a.u[3], lbl, lbr, a.p[0],
+                                       a.p[1], a.p[2], a.p[3]);
+#pragma unroll
+            for (int k = 0; k < L; ++k) {
+                qLb[k] = qLb[k] + sL * lbl[k];
+                qRb[k] = qRb[k] + sR * lbr[k];
+                a.c[0][k] = a.c[0][k] + lbl[k] * sn;
+                a.c[1][k] = a.c[1][k] + lbr[k] * sn;
+            }
+        }
+        if (dissR) {
+            // R's view: lam of q' = -q with R's own normal (signs of exact zeros from nR)
+            const double sL = -sign1(val(nR[0]) * val(Lc[1]) + val(nR[1]) * val(Lc[2]));
+            const double sR = -sign1(val(nR[0]) * val(R[1]) + val(nR[1]) * val(R[2]));
+            const T lamL = sabs(qL) + cL * sn, lamR = sabs(qR) + cR * sn;
+            T db[4][L], lbl[L], lbr[L];
+#pragma unroll
+            for (int k = 0; k < L; ++k) {
+                lbl[k] = lbr[k] = lit<T>(0.0);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) db[v][k] = -fbR[v][k];
+            }
+            side_dissipation_rev<T, L>(P, lamR, lamL, pRR, pR, pL, pLL, RR, R, Lc, LL, db,
+                                       a.u[3], a.u[2], a.u[1], a.u[0], lbr, lbl, a.p[3],
+                                       a.p[2], a.p[1], a.p[0]);
+#pragma unroll
+            for (int k = 0; k < L; ++k) {
+                qLb[k] = qLb[k] + sL * lbl[k];
+                qRb[k] = qRb[k] + sR * lbr[k];
+                a.c[0][k] = a.c[0][k] + lbl[k] * sn;
+                a.c[1][k] = a.c[1][k] + lbr[k] * sn;
+            }
+        }
+    }
+    // q_X = (n0 X1 + n1 X2) / X0
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+        const T tl = qLb[k] * iL, tr = qRb[k] * iR;
+        a.u[1][1][k] = a.u[1][1][k] + tl * n[0];
+        a.u[1][2][k] = a.u[1][2][k] + tl * n[1];
+        a.u[1][0][k] = a.u[1][0][k] - tl * qL;
+        a.u[2][1][k] = a.u[2][1][k] + tr * n[0];
+        a.u[2][2][k] = a.u[2][2][k] + tr * n[1];
+        a.u[2][0][k] = a.u[2][0][k] - tr * qR;
+    }
+}
+
 // Per-cell pressure and sound speed sqrt(gamma p / rho) (f90:128-133, 147).
 template <class T>
 PEA_HD void cell_pc(const T* u, T& p, T& c) {

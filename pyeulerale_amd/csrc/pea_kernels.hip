@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 
 #include "pea_adjoint.hpp"
 #include "pea_face.hpp"
@@ -1140,6 +1141,307 @@ k_vjp(pea_params P, Ld ld, const double* __restrict__ wseed, double* __restrict_
     }
 }
 
+// ------------------------------------------------------------------ face-based adjoint
+// (dR/dU)^T w for a TM x TN tile of cells (T = double, two lanes: real and imaginary part of
+// the seed).  Every face whose stencil reaches the tile is reversed once for both of its cells
+// (face_flux2_rev): first the angular faces of the tile's rows, then the radial faces of its
+// columns.  A face's contributions to its four cells along one grid line are added to LDS
+// accumulators in four barrier-separated phases (offset -1, 0, +1, +2 from L), so within a
+// phase every face writes a distinct cell and the order of every sum is fixed.  Pressure and
+// sound-speed adjoints are accumulated as scalars and turned into state adjoints once per cell
+// at the end, together with the wall / far-field faces and their ghost states.
+// Requires TN + 3 <= N (no face is reached twice through the periodic seam).
+template <int TM, int TN, int NT>
+__global__ void __launch_bounds__(NT)
+k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __restrict__ gU) {
+    using T = double;
+    constexpr int L = 2;
+    constexpr int BM = TM + 6, BN = TN + 6, BOX = BM * BN, NC = TM * TN;
+    __shared__ T sP[BOX], sC[BOX];
+    __shared__ T sFB[BOX][4][L];          // -w / area of each cell of the grown tile
+    __shared__ T sG[2][BN][4];            // wall (m = -1) and far-field (m = M) ghosts
+    __shared__ T accU[NC][4][L], accP[NC][L], accC[NC][L];
+    __shared__ T gU_[2][TN][4][L], gP_[2][TN][L];   // ghost adjoints of the tile's columns
+    const Grid g = ld.g;
+    const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
+    const int tid = threadIdx.x;
+
+    auto vtx = [&](int mm, int nn, T* x) { ld.vertex(nn * (g.M + 1) + mm, x); };
+    auto nrm = [&](int ma, int na, int mb, int nb, T* nr) {
+        T xa[2], xb[2];
+        vtx(ma, na, xa);
+        vtx(mb, nb, xb);
+        edge_normal(xa, xb, nr);
+    };
+    auto edge = [&](int ma, int na, int mb, int nb, T* nr, T* gv) {
+        T va[2], vb[2];
+        nrm(ma, na, mb, nb, nr);
+        ld.velocity(na * (g.M + 1) + ma, va);
+        ld.velocity(nb * (g.M + 1) + mb, vb);
+        gv[0] = (va[0] + vb[0]) / 2.0;
+        gv[1] = (va[1] + vb[1]) / 2.0;
+    };
+    auto bidx = [&](int mm, int nn_box) { return nn_box * BM + (mm - m0 + 3); };
+    // box column of cell column nn (wrapped), for nn within three columns of the tile
+    auto bcol = [&](int nn) {
+        int d = nn - (n0 - 3);
+        if (d < 0) d += g.N;
+        if (d >= g.N) d -= g.N;
+        return d;
+    };
+    auto in_tile_n = [&](int nn) { return nn >= n0 && nn < n0 + TN; };
+
+    // 1. per-cell pressures, sound speeds, seeds; ghosts; zero accumulators
+    for (int i = tid; i < BOX; i += NT) {
+        const int bm = i % BM, bn = i / BM;
+        const int cm = m0 - 3 + bm;
+        const int cn = ((n0 - 3 + bn) % g.N + g.N) % g.N;
+        if (cm >= 0 && cm < g.M) {
+            T u[4];
+            const int c = cn * g.M + cm;
+            ld.state(c, u);
+            cell_pc(u, sP[i], sC[i]);
+            T x[4][2];
+            vtx(cm, cn, x[BI]);
+            vtx(cm, cn + 1, x[FI]);
+            vtx(cm + 1, cn + 1, x[FO]);
+            vtx(cm + 1, cn, x[BO]);
+            const T d1x = x[FO][0] - x[BI][0], d1z = x[FO][1] - x[BI][1];
+            const T d2x = x[BO][0] - x[FI][0], d2z = x[BO][1] - x[FI][1];
+            const T ia = 1.0 / (sabs(d1x * d2z - d1z * d2x) / 2.0);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                sFB[i][v][0] = -(wseed[2 * ((long)v * g.C + c)] * ia);
+                sFB[i][v][1] = -(wseed[2 * ((long)v * g.C + c) + 1] * ia);
+            }
+        } else if (cm == -1 || cm == g.M) {
+            const int mc = cm < 0 ? 0 : g.M - 1;
+            T u[4], gh[4];
+            ld.state(cn * g.M + mc, u);
+            if (cm < 0) {
+                T nr[2], gv[2];
+                edge(0, cn, 0, cn + 1, nr, gv);
+                wall_ghost(u, nr, gv, gh);
+            } else {
+                T inf[4];
+                free_stream(P, ld.mach, inf);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) gh[v] = 2.0 * inf[v] - u[v];
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sG[cm < 0 ? 0 : 1][bn][v] = gh[v];
+            sP[i] = pressure(gh);
+        }
+    }
+    for (int i = tid; i < NC * 4 * L; i += NT) (&accU[0][0][0])[i] = 0.0;
+    for (int i = tid; i < NC * L; i += NT) {
+        (&accP[0][0])[i] = 0.0;
+        (&accC[0][0])[i] = 0.0;
+    }
+    for (int i = tid; i < 2 * TN * 4 * L; i += NT) (&gU_[0][0][0][0])[i] = 0.0;
+    for (int i = tid; i < 2 * TN * L; i += NT) (&gP_[0][0][0])[i] = 0.0;
+    __syncthreads();
+
+    auto add = [&](int t, const FaceAdj<T, L>& a, int j) {
+#pragma unroll
+        for (int k = 0; k < L; ++k) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) accU[t][v][k] += a.u[j][v][k];
+            accP[t][k] += a.p[j][k];
+            if (j == 1 || j == 2) accC[t][k] += a.c[j - 1][k];
+        }
+    };
+
+    // 2. angular faces (m, Lc) | (m, Lc+1) for the tile's rows, Lc from n0-2 to n0+TN
+    {
+        constexpr int NF = TM * (TN + 3);
+        for (int base = 0; base < NF; base += NT) {
+            const int f = base + tid;
+            const int tm = f % TM, j = f / TM;
+            const int m = m0 + tm;
+            const bool act = f < NF && m < g.M;
+            const int Lc = ((n0 - 2 + j) % g.N + g.N) % g.N;
+            FaceAdj<T, L> a;
+            if (act) {
+                face_adj_zero(a);
+                const int Rc = wrapn(Lc + 1, g.N), LLc = wrapn(Lc - 1, g.N),
+                          RRc = wrapn(Lc + 2, g.N);
+                T Ls[4], Rs[4], LLs[4], RRs[4];
+                ld.state(Lc * g.M + m, Ls);
+                ld.state(Rc * g.M + m, Rs);
+                ld.state(LLc * g.M + m, LLs);
+                ld.state(RRc * g.M + m, RRs);
+                const int bL = bidx(m, bcol(Lc)), bR = bidx(m, bcol(Rc));
+                const int bLL = bidx(m, bcol(LLc)), bRR = bidx(m, bcol(RRc));
+                T nL[2], nRn[2], gv[2];
+                edge(m, Lc + 1, m + 1, Lc + 1, nL, gv);     // L's fi -> fo
+                nrm(m + 1, Rc, m, Rc, nRn);                 // R's bo -> bi
+                const T zero[4][L] = {};
+                if (Lc + 1 == g.N) {
+                    // seam: L's face uses vertex column N, R's face column 0 -- reverse each
+                    // side with its own geometry
+                    face_flux2_rev<T, T, L>(P, LLs, Ls, Rs, RRs, sP[bLL], sP[bL], sP[bR],
+                                            sP[bRR], sC[bL], sC[bR], nL, nRn, gv, true, false,
+                                            sFB[bL], zero, a);
+                    T nL0[2], gv0[2];
+                    edge(m, 0, m + 1, 0, nL0, gv0);
+                    face_flux2_rev<T, T, L>(P, LLs, Ls, Rs, RRs, sP[bLL], sP[bL], sP[bR],
+                                            sP[bRR], sC[bL], sC[bR], nL0, nRn, gv0, false, true,
+                                            zero, sFB[bR], a);
+                } else {
+                    face_flux2_rev<T, T, L>(P, LLs, Ls, Rs, RRs, sP[bLL], sP[bL], sP[bR],
+                                            sP[bRR], sC[bL], sC[bR], nL, nRn, gv, true, true,
+                                            sFB[bL], sFB[bR], a);
+                }
+            }
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                if (act && tm < TM) {
+                    const int nt = wrapn(Lc - 1 + jj, g.N);
+                    if (in_tile_n(nt) && nt < g.N) add((nt - n0) * TM + tm, a, jj);
+                }
+                __syncthreads();
+            }
+        }
+    }
+    // 3. radial faces (Lm, n) | (Lm+1, n) for the tile's columns, Lm from m0-2 to m0+TM
+    {
+        constexpr int NF = (TM + 3) * TN;
+        for (int base = 0; base < NF; base += NT) {
+            const int f = base + tid;
+            const int i = f % (TM + 3), tn = f / (TM + 3);
+            const int n = n0 + tn, Lm = m0 - 2 + i;
+            const bool act = f < NF && n < g.N && Lm >= 0 && Lm <= g.M - 2;
+            FaceAdj<T, L> a;
+            if (act) {
+                face_adj_zero(a);
+                const int bn = tn + 3;
+                T Ls[4], Rs[4], LLs[4], RRs[4];
+                const int c = n * g.M + Lm;
+                ld.state(c, Ls);
+                ld.state(c + 1, Rs);
+                if (Lm == 0) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) LLs[v] = sG[0][bn][v];
+                } else {
+                    ld.state(c - 1, LLs);
+                }
+                if (Lm + 1 == g.M - 1) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) RRs[v] = sG[1][bn][v];
+                } else {
+                    ld.state(c + 2, RRs);
+                }
+                T nL[2], nRn[2], gv[2];
+                edge(Lm + 1, n + 1, Lm + 1, n, nL, gv);      // L's fo -> bo
+                nrm(Lm + 1, n, Lm + 1, n + 1, nRn);          // R's bi -> fi
+                face_flux2_rev<T, T, L>(P, LLs, Ls, Rs, RRs, sP[bidx(Lm - 1, bn)],
+                                        sP[bidx(Lm, bn)], sP[bidx(Lm + 1, bn)],
+                                        sP[bidx(Lm + 2, bn)], sC[bidx(Lm, bn)],
+                                        sC[bidx(Lm + 1, bn)], nL, nRn, gv, Lm != g.M - 2,
+                                        Lm + 1 != 1, sFB[bidx(Lm, bn)], sFB[bidx(Lm + 1, bn)], a);
+            }
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                if (act) {
+                    const int mt = Lm - 1 + jj;
+                    if (mt == -1) {
+                        if (m0 == 0) {
+#pragma unroll
+                            for (int k = 0; k < L; ++k) {
+#pragma unroll
+                                for (int v = 0; v < 4; ++v) gU_[0][tn][v][k] += a.u[jj][v][k];
+                                gP_[0][tn][k] += a.p[jj][k];
+                            }
+                        }
+                    } else if (mt == g.M) {
+                        if (m0 + TM >= g.M) {
+#pragma unroll
+                            for (int k = 0; k < L; ++k) {
+#pragma unroll
+                                for (int v = 0; v < 4; ++v) gU_[1][tn][v][k] += a.u[jj][v][k];
+                                gP_[1][tn][k] += a.p[jj][k];
+                            }
+                        }
+                    } else if (mt >= m0 && mt < m0 + TM) {
+                        add(tn * TM + (mt - m0), a, jj);
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+    // 4. per cell: wall / far-field faces and ghosts, then pressure and sound-speed adjoints
+    for (int t = tid; t < NC; t += NT) {
+        const int tm = t % TM, tn = t / TM;
+        const int m = m0 + tm, n = n0 + tn;
+        if (m >= g.M || n >= g.N) continue;
+        const int c = n * g.M + m, bn = tn + 3, b = bidx(m, bn);
+        T u[4];
+        ld.state(c, u);
+        T ub[4][L], pb[L], cb[L];
+#pragma unroll
+        for (int k = 0; k < L; ++k) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) ub[v][k] = accU[t][v][k];
+            pb[k] = accP[t][k];
+            cb[k] = accC[t][k];
+        }
+        if (m == 0 || m == g.M - 1) {
+            const int side = m == 0 ? 0 : 1;
+            T gh[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) gh[v] = sG[side][bn][v];
+            T nr[2], gv[2];
+            if (side == 0) edge(0, n, 0, n + 1, nr, gv);           // wall face bi -> fi
+            else edge(m + 1, n + 1, m + 1, n, nr, gv);             // far field fo -> bo
+            // central face (cell, ghost) seen from the cell: f_L of face_flux2 with L = cell
+            FaceAdj<T, L> a;
+            face_adj_zero(a);
+            const T zero[4][L] = {};
+            face_flux2_rev<T, T, L>(P, u, u, gh, gh, sP[b], sP[b], sP[bidx(side ? g.M : -1, bn)],
+                                    sP[bidx(side ? g.M : -1, bn)], sC[b], sC[b], nr, nr, gv,
+                                    false, false, sFB[b], zero, a);
+            T ghb[4][L], ghp[L];
+#pragma unroll
+            for (int k = 0; k < L; ++k) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    ub[v][k] += a.u[1][v][k];
+                    ghb[v][k] = gU_[side][tn][v][k] + a.u[2][v][k];
+                }
+                pb[k] += a.p[1][k];
+                ghp[k] = gP_[side][tn][k] + a.p[2][k];
+            }
+            pressure_rev<T, L>(gh, 1.0 / gh[0], ghp, ghb);
+            if (side == 0) {
+                T nb_[2][L], gvb_[2][L];
+                wall_ghost_rev<T, L>(u, nr, gv, ghb, ub, nb_, gvb_);
+            } else {
+#pragma unroll
+                for (int k = 0; k < L; ++k)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) ub[v][k] -= ghb[v][k];
+            }
+        }
+        // c = sqrt(gamma p / rho)
+        const T p = sP[b], cc = sC[b], iu = 1.0 / u[0];
+        const T a_ = kGamma * p / u[0];
+#pragma unroll
+        for (int k = 0; k < L; ++k) {
+            const T ab = cb[k] / (2.0 * cc);
+            pb[k] += kGamma * ab * iu;
+            ub[0][k] -= ab * a_ * iu;
+        }
+        pressure_rev<T, L>(u, iu, pb, ub);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            gU[2 * ((long)v * g.C + c)] = ub[v][0];
+            gU[2 * ((long)v * g.C + c) + 1] = ub[v][1];
+        }
+    }
+}
+
 // Sums per-block complex partials in block order (deterministic).
 __global__ void __launch_bounds__(256)
 k_sum_partials(const double* __restrict__ partial, int count, double* __restrict__ out) {
@@ -1211,6 +1513,7 @@ std::mutex g_tile_mutex;
 TileChoice g_tiles[64];
 int g_ntiles = 0;
 int g_tile_global = -1;
+bool g_face_vjp = true;   // face-based adjoint for the states-only product
 
 // Default: minimise the estimated time max(rounds, blocks*rounds/512) -- 512 resident blocks
 // over 256 CUs -- then the total work.
@@ -1252,9 +1555,20 @@ int launch_vjp_shape(const pea_params& P, const Ld& ld, const double* w, double*
     return PEA_OK;
 }
 
+constexpr int kVjpFaceTM = 32, kVjpFaceTN = 5;
+
 template <class T, class Ld>
 int launch_vjp(const pea_params& P, const Ld& ld, const double* w, double* gU, double* gX,
                double* gV, double* gmach, void* ws, hipStream_t st) {
+    if constexpr (std::is_same<T, double>::value && std::is_same<Ld, LdReal>::value) {
+        if (gU && !gX && !gV && !gmach && ld.g.N >= kVjpFaceTN + 3 && g_face_vjp) {
+            const dim3 grid((ld.g.M + kVjpFaceTM - 1) / kVjpFaceTM,
+                            (ld.g.N + kVjpFaceTN - 1) / kVjpFaceTN);
+            k_face_vjp<kVjpFaceTM, kVjpFaceTN, 256><<<grid, 256, 0, st>>>(P, ld, w, gU);
+            PEA_LAUNCH_CHECK("k_face_vjp");
+            return PEA_OK;
+        }
+    }
     constexpr bool small = sizeof(T) > sizeof(double);
     const int tile = pick_tile(ld.g, small);
     if (tile < 0) return fail(PEA_ERR_DIMS, "no adjoint tile fits N=%d", ld.g.N);

@@ -130,9 +130,13 @@ def test_jvp_vjp_against_complex_step_blocks(name, orc, K):
     for key in ref:
         got = out[key].cpu().numpy() if key == "mach" else host(out[key])
         assert rel(got, ref[key]) <= TOL_JAC, (key, rel(got, ref[key]))
-    # the U-only variant of the adjoint kernel gives the same bits
+    # the states-only adjoint (face-based kernel) agrees with the all-outputs kernel and is
+    # bitwise reproducible (fixed accumulation order, no atomics)
     only = K.vjp_odes(p, mach, g, Xd, Vd, Ud, dev_cells(w), want=("states",))
-    assert torch.equal(only["states"], out["states"])
+    assert rel(host(only["states"]), ref["states"]) <= TOL_JAC
+    assert rel(host(only["states"]), host(out["states"])) <= 1e-13
+    again = K.vjp_odes(p, mach, g, Xd, Vd, Ud, dev_cells(w), want=("states",))
+    assert torch.equal(only["states"], again["states"])
 
 
 @pytest.mark.parametrize("name", ["tiny", "small_pitch"])
