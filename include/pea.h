@@ -175,6 +175,48 @@ int pea_apply_odes_wrt_vertices_rev(int M, int N, const double *JX, const double
 int pea_export_bsr(int M, int N, const double *JU, double shift_re, double shift_im,
                    int complex_out, double *data, int *indices, int *indptr, void *stream);
 
+/* ------------------------------------------------------------------ resolvent kernels */
+/* Building blocks of the batched resolvent (dR/dU - s I)^-1 for S shifts at once (the
+ * reference factorises one shift at a time with SuperLU, core.py:533-599).  `shifts` holds S
+ * complex values on the device; vectors are S consecutive complex cell fields.  With
+ * transposed != 0 every operator is that of the adjoint system (dR/dU)^T - conj(s) I
+ * (core.py:594-598).  JU are the assembled blocks of pea_cs_linearize_odes. */
+const char *pea_resolvent_last_error(void);
+
+/* y_k = (J - s_k I) x_k for k < S, reading the blocks once for all vectors. */
+int pea_block_multi(int M, int N, int S, const double *JU, const double *shifts,
+                    int transposed, const double *x, double *y, void *stream);
+
+/* Radial-line preconditioner: block LU of the block-pentadiagonal matrix of each radial line
+ * (blocks ii, i, c - sI, o, oo) for every shift; F needs pea_line_factor_bytes(M, N, S). */
+long long pea_line_factor_bytes(int M, int N, int S);
+int pea_line_factor(int M, int N, int S, const double *JU, const double *shifts,
+                    int transposed, double *F, void *stream);
+int pea_line_solve(int M, int N, int S, const double *JU, const double *F, int transposed,
+                   const double *r, double *x, void *stream);
+
+/* Exact direct resolvent by block LU over radial rings (replaces the SuperLU factorisation of
+ * core.py:562-564 and 596-598).  Ring k holds every cell at radial index k; its 4N unknowns
+ * are ordered (n, v).  The system is block pentadiagonal over rings: a dense ring block and
+ * couplings to rings k-2..k+2 that are block diagonal in n.
+ * pea_ring_band: D[s] (S x 4N x 4N complex, row-major) = ring block k of the system, the
+ *   shift on the diagonal, zero outside the periodic band.
+ * pea_ring_couple: Z[s] = beta Z[s] + alpha C Y[s], C = coupling of ring k to ring k + dk
+ *   (dk in -2, -1, 1, 2); Y, Z are S x 4N x q complex; Y = NULL stands for the identity
+ *   (q = 4N), which writes C as a dense matrix.
+ * The eliminations between these calls are dense complex GEMM and LU. */
+int pea_ring_band(int M, int N, int S, const double *JU, const double *shifts, int transposed,
+                  int k, double *D, void *stream);
+int pea_ring_couple(int M, int N, int S, const double *JU, int transposed, int k, int dk, int q,
+                    double alpha, const double *Y, double beta, double *Z, void *stream);
+
+/* Block-Jacobi preconditioner: inverses of the 4x4 diagonal blocks (J_cc - s I);
+ * F holds S*16*M*N complex values. */
+int pea_jacobi_factor(int M, int N, int S, const double *JU, const double *shifts,
+                      int transposed, double *F, void *stream);
+int pea_jacobi_solve(int M, int N, int S, const double *F, const double *r, double *x,
+                     void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,8 +50,21 @@ _SIGS = {
     "pea_apply_odes_wrt_vertices_fwd": [_I, _I, _P, _P, _P, _P],
     "pea_apply_odes_wrt_vertices_rev": [_I, _I, _P, _P, _P, _P],
     "pea_export_bsr": [_I, _I, _P, _D, _D, _I, _P, _P, _P, _P],
+    "pea_resolvent_last_error": [],
+    "pea_block_multi": [_I, _I, _I, _P, _P, _I, _P, _P, _P],
+    "pea_line_factor_bytes": [_I, _I, _I],
+    "pea_line_factor": [_I, _I, _I, _P, _P, _I, _P, _P],
+    "pea_line_solve": [_I, _I, _I, _P, _P, _I, _P, _P, _P],
+    "pea_jacobi_factor": [_I, _I, _I, _P, _P, _I, _P, _P],
+    "pea_jacobi_solve": [_I, _I, _I, _P, _P, _P, _P],
+    "pea_ring_band": [_I, _I, _I, _P, _P, _I, _I, _P, _P],
+    "pea_ring_couple": [_I, _I, _I, _P, _I, _I, _I, _I, _D, _P, _D, _P, _P],
 }
-_RESTYPES = {"pea_last_error": _c.c_char_p, "pea_vjp_workspace_bytes": _c.c_longlong}
+_RESTYPES = {"pea_last_error": _c.c_char_p, "pea_vjp_workspace_bytes": _c.c_longlong,
+             "pea_resolvent_last_error": _c.c_char_p, "pea_line_factor_bytes": _c.c_longlong}
+
+_RESOLVENT = {"pea_block_multi", "pea_line_factor", "pea_line_solve", "pea_jacobi_factor",
+              "pea_jacobi_solve", "pea_ring_band", "pea_ring_couple"}
 
 _lib = None
 
@@ -80,6 +93,7 @@ def call(name: str, *args) -> int:
     """Calls ``name`` and raises :class:`PeaError` on a negative status."""
     rc = getattr(load(), name)(*args)
     if rc < 0:
-        msg = load().pea_last_error().decode(errors="replace")
+        src = load().pea_resolvent_last_error if name in _RESOLVENT else load().pea_last_error
+        msg = src().decode(errors="replace")
         raise PeaError(f"{name} failed ({rc}): {msg}")
     return rc

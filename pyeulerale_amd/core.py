@@ -14,8 +14,9 @@ Same constructor keywords, properties and method names as the reference class
   attack, JST constants).  The ``apply_*`` products then use hand-written tangent / adjoint
   kernels at that point (``jacobian_mode="matrix_free"``, default) or complex-step assembled
   blocks and block products, the reference's method (``jacobian_mode="complex_step"``);
-* the resolvent is matrix-free GMRES on the GPU (``solver="gmres"``) or the reference's direct
-  sparse LU on the exported block matrix (``solver="direct"``);
+* the resolvent is solved on the GPU, exactly by block LU over radial rings (``solver=
+  "direct"``, many shifts batched) or by batched matrix-free GMRES (``solver="gmres"``); the
+  reference's host sparse LU on the exported block matrix stays available (``"superlu"``);
 * second-order gradients are exact tangent-over-adjoint products instead of central finite
   differences of complex-step Jacobians (core.py:718-1275).
 """
@@ -92,6 +93,7 @@ class SpatialDiscretization:
         self._F = torch.zeros(self._grid.force_shape, dtype=F64, device=self.device)
         self._lin = None
         self._blocks = None
+        self._factors = None
         self.set_free_stream_state()
 
     # ------------------------------------------------------------------ properties
@@ -229,6 +231,7 @@ class SpatialDiscretization:
         self._lin = dict(p=self._params(), mach=self._mach, X=self._X.clone(),
                          V=self._V.clone(), U=self._U.clone())
         self._blocks = None
+        self._factors = None
         if self.jacobian_mode == "complex_step":
             self._assemble_blocks()
 
@@ -395,7 +398,11 @@ class SpatialDiscretization:
     # ------------------------------------------------------------------ resolvent
     def solve_odes_wrt_states_fwd(self, d_odes, shift: complex = 0.0, d_states=None,
                                   solver: str = "direct", **opts) -> torch.Tensor:
-        """``(dR/dU - shift I)^-1 . d_odes`` at the linearization point (core.py:533-565)."""
+        """``(dR/dU - shift I)^-1 . d_odes`` at the linearization point (core.py:533-565).
+
+        ``solver``: ``"direct"`` (exact block LU over radial rings on the GPU; the factorisation
+        is kept for further solves with the same shift until the next ``linearize``),
+        ``"superlu"`` (the reference's sparse LU on the host) or ``"gmres"``."""
         return self._solve(d_odes, complex(shift), False, d_states, solver, opts)
 
     def solve_odes_wrt_states_adj(self, d_states, shift: complex = 0.0, d_odes=None,
@@ -404,14 +411,67 @@ class SpatialDiscretization:
         ``(dR/dU)^T - conj(shift) I``."""
         return self._solve(d_states, complex(shift), True, d_odes, solver, opts)
 
+    def solve_odes_wrt_states_batched(self, d_odes, shifts, adjoint: bool = False,
+                                      solver: str = "direct", fallback: str | None = "direct",
+                                      **opts):
+        """All shifts at once: x_k = (dR/dU - s_k I)^-1 d_odes_k (or the adjoint).
+        d_odes of shape (S, NUM_VAR, M, N); returns the same shape (views).
+
+        ``solver="direct"`` factorises all shifts together (batched ring LU, in chunks that fit
+        ``max_bytes`` of device memory).  ``solver="gmres"`` runs batched GMRES; shifts whose
+        residual is still above ``rtol`` after ``maxiter`` are solved again with the direct
+        method when ``fallback="direct"`` (the default), or left as they are (``fallback=None``;
+        ``last_gmres.converged`` tells which)."""
+        t = d_odes if isinstance(d_odes, torch.Tensor) else _as_tensor(d_odes, C128, self.device)
+        shifts = np.asarray(shifts, dtype=complex).reshape(-1)
+        S = len(shifts)
+        if tuple(t.shape) != (S, NUM_VAR, self.num_radial, self.num_angular):
+            raise RuntimeError(f"d_odes: incorrect shape {tuple(t.shape)}")
+        if fallback not in ("direct", None):
+            raise ValueError(f"unknown fallback {fallback!r}")
+        b = K.cells_in(t, C128, self.device)
+        p, mach, g, X, V, U = self._point()
+        if solver == "direct":
+            from .ringlu import solve_shifts
+            x, self.last_residual = solve_shifts(g, self._get_blocks()["U"], b, shifts,
+                                                 adjoint=adjoint, **opts)
+        elif solver == "gmres":
+            from .gmres import BatchedGMRES
+            gm = BatchedGMRES.from_discretization(self, shifts, adjoint=adjoint, **opts)
+            x = gm.solve(b)
+            self.last_gmres = gm.info
+            todo = np.flatnonzero(~gm.info.converged)
+            if fallback == "direct" and todo.size:
+                from .ringlu import solve_shifts
+                x[todo], _ = solve_shifts(g, self._get_blocks()["U"], b[todo], shifts[todo],
+                                          adjoint=adjoint)
+        else:
+            raise ValueError(f"unknown solver {solver!r}")
+        return x.transpose(-1, -2)
+
+    def _ring_factors(self, shift, adjoint):
+        from .ringlu import RingLU
+        key = (complex(shift), bool(adjoint))
+        if self._factors is None or self._factors[0] != key:
+            self._factors = None        # release the previous factorisation first
+            p, mach, g, X, V, U = self._point()
+            self._factors = (key, RingLU(g, self._get_blocks()["U"], [shift], adjoint=adjoint))
+        return self._factors[1]
+
     def _solve(self, rhs, shift, adjoint, out, solver, opts):
         b = self._cells(rhs, "rhs")
         if solver == "direct":
+            lu = self._ring_factors(shift, adjoint)
+            x = lu.solve(b[None])[0]
+            self.last_residual = lu.residual
+        elif solver == "superlu":
             from .resolvent import direct_solve
             x = direct_solve(self, b, shift, adjoint)
         elif solver == "gmres":
-            from .resolvent import gmres_solve
-            x = gmres_solve(self, b[None], [shift], adjoint, **opts)[0]
+            from .gmres import BatchedGMRES
+            gm = BatchedGMRES.from_discretization(self, [shift], adjoint=adjoint, **opts)
+            x = gm.solve(b[None])[0]
+            self.last_gmres = gm.info
         else:
             raise ValueError(f"unknown solver {solver!r}")
         res = self._cell_out(x)

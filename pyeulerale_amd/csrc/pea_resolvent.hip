@@ -1,0 +1,590 @@
+// Kernels of the resolvent (dR/dU - s I)^-1 for many shifts s at once.
+//
+// * k_block_fwd_multi / k_block_rev_multi: products of the assembled dR/dU blocks with S complex
+//   vectors (operator of batched GMRES).  A workgroup stages the 9 blocks of 32 cells in LDS once
+//   and applies them to all vectors, so the 1152 B/cell of blocks are read once per product
+//   batch instead of once per vector.  The shift is applied in the same pass: y = J x - s x.
+// * k_line_factor / k_line_solve: radial-line preconditioner.  For every angular index n and
+//   shift s the block-pentadiagonal matrix of the radial line (blocks ii, i, c - sI, o, oo) is
+//   factorised by block LU (partial pivoting inside each 4x4 diagonal block) and solved.
+//   One thread per (line, shift), m sequential.  The transposed variant factorises the line of
+//   (dR/dU)^T - conj(s) I for the adjoint solve.
+// * k_jacobi_factor / k_jacobi_solve: block-Jacobi preconditioner (4x4 diagonal blocks).
+// * k_ring_band / k_ring_couple: pieces of the exact block-banded LU over radial rings.  A ring
+//   is every cell at one radial index m; ordered (n, v) its unknowns form a dense 4N x 4N
+//   block.  The matrix is block pentadiagonal in m: the ring's own block (blocks c, b, bb, f,
+//   ff, periodic in n, minus the shift) and four couplings to rings m-2..m+2 that are
+//   block diagonal in n (blocks ii, i, o, oo).  k_ring_band writes the dense ring block for all
+//   shifts; k_ring_couple forms Z = beta Z + alpha C Y with a coupling C, or C itself as a dense
+//   matrix when Y is null.  The dense eliminations between them are library GEMM/LU calls.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+
+#include "../../include/pea.h"
+
+namespace {
+
+struct Z {
+    double r, i;
+};
+__device__ __forceinline__ Z zadd(Z a, Z b) { return {a.r + b.r, a.i + b.i}; }
+__device__ __forceinline__ Z zsub(Z a, Z b) { return {a.r - b.r, a.i - b.i}; }
+__device__ __forceinline__ Z zmul(Z a, Z b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+__device__ __forceinline__ Z zfma(Z a, Z b, Z c) {   // a*b + c
+    return {fma(a.r, b.r, fma(-a.i, b.i, c.r)), fma(a.r, b.i, fma(a.i, b.r, c.i))};
+}
+__device__ __forceinline__ Z zinv(Z a) {
+    const double d = a.r * a.r + a.i * a.i;
+    return {a.r / d, -a.i / d};
+}
+__device__ __forceinline__ double zabs2(Z a) { return a.r * a.r + a.i * a.i; }
+
+// 4x4 complex block in registers
+struct B4 {
+    Z a[4][4];
+};
+
+__device__ __forceinline__ B4 bmul(const B4& x, const B4& y) {
+    B4 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            Z s = {0.0, 0.0};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) s = zfma(x.a[i][k], y.a[k][j], s);
+            r.a[i][j] = s;
+        }
+    return r;
+}
+__device__ __forceinline__ B4 bsub(const B4& x, const B4& y) {
+    B4 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) r.a[i][j] = zsub(x.a[i][j], y.a[i][j]);
+    return r;
+}
+// inverse by Gauss-Jordan with partial pivoting
+__device__ B4 binv(const B4& x) {
+    Z m[4][8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            m[i][j] = x.a[i][j];
+            m[i][j + 4] = Z{i == j ? 1.0 : 0.0, 0.0};
+        }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        int p = c;
+        double best = zabs2(m[c][c]);
+#pragma unroll
+        for (int r = c + 1; r < 4; ++r)
+            if (zabs2(m[r][c]) > best) {
+                best = zabs2(m[r][c]);
+                p = r;
+            }
+        if (p != c)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                Z t = m[c][j];
+                m[c][j] = m[p][j];
+                m[p][j] = t;
+            }
+        const Z piv = zinv(m[c][c]);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) m[c][j] = zmul(m[c][j], piv);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            if (r != c) {
+                const Z f = m[r][c];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) m[r][j] = zsub(m[r][j], zmul(f, m[c][j]));
+            }
+    }
+    B4 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) r.a[i][j] = m[i][j + 4];
+    return r;
+}
+
+// block s of cell (m, n) of the assembled Jacobian (real), optionally transposed, minus
+// `shift` on the diagonal
+__device__ __forceinline__ B4 load_jblock(const double* JU, int C, int cell, int s, bool tr,
+                                          Z shift) {
+    B4 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int ii = tr ? j : i, jj = tr ? i : j;
+            r.a[i][j] = Z{JU[(long)((s * 4 + ii) * 4 + jj) * C + cell], 0.0};
+            if (i == j) r.a[i][j] = zsub(r.a[i][j], shift);
+        }
+    return r;
+}
+
+// factor storage: per (shift, m, slot, i, j) a plane over n:  F[(((sh*M + m)*4 + slot)*16
+// + 4i + j)*N + n], slots 0 = L2, 1 = L1, 2 = Dinv, 3 = U1
+__device__ __forceinline__ long fidx(int M, int N, int sh, int m, int slot, int ij, int n) {
+    return ((((long)sh * M + m) * 4 + slot) * 16 + ij) * N + n;
+}
+__device__ __forceinline__ void store_b(double* F, int M, int N, int sh, int m, int slot, int n,
+                                        const B4& b) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const long e = fidx(M, N, sh, m, slot, 4 * i + j, n);
+            F[2 * e] = b.a[i][j].r;
+            F[2 * e + 1] = b.a[i][j].i;
+        }
+}
+__device__ __forceinline__ B4 load_b(const double* F, int M, int N, int sh, int m, int slot,
+                                     int n) {
+    B4 b;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const long e = fidx(M, N, sh, m, slot, 4 * i + j, n);
+            b.a[i][j] = Z{F[2 * e], F[2 * e + 1]};
+        }
+    return b;
+}
+
+// Radial neighbour blocks of row m on the line: ii (m-2), i (m-1), c, o (m+1), oo (m+2).
+// Transposed system: block (m, m+k) of J^T is block (m+k, m) of J transposed, i.e. the block
+// of cell m+k whose neighbour slot points back at m.
+__device__ __forceinline__ B4 line_block(const double* JU, int M, int N, int n, int m, int off,
+                                         bool tr, Z shift) {
+    const int C = M * N;
+    // slots: ii = 6, i = 5, c = 0, o = 7, oo = 8 ; reverse slot of an offset
+    if (!tr) {
+        const int s = off == -2 ? 6 : off == -1 ? 5 : off == 0 ? 0 : off == 1 ? 7 : 8;
+        return load_jblock(JU, C, n * M + m, s, false, off == 0 ? shift : Z{0.0, 0.0});
+    }
+    const int s = off == -2 ? 8 : off == -1 ? 7 : off == 0 ? 0 : off == 1 ? 5 : 6;
+    return load_jblock(JU, C, n * M + m + off, s, true, off == 0 ? shift : Z{0.0, 0.0});
+}
+
+__global__ void __launch_bounds__(64)
+k_line_factor(int M, int N, int S, const double* __restrict__ JU, const double* __restrict__ sh,
+              int tr, double* __restrict__ F) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (n >= N || s >= S) return;
+    // adjoint: (J - s I)^H = J^T - conj(s) I
+    const Z shift = tr ? Z{sh[2 * s], -sh[2 * s + 1]} : Z{sh[2 * s], sh[2 * s + 1]};
+    for (int m = 0; m < M; ++m) {
+        B4 D = line_block(JU, M, N, n, m, 0, tr, shift);
+        B4 L2, L1;
+        if (m >= 2) {
+            L2 = bmul(line_block(JU, M, N, n, m, -2, tr, shift), load_b(F, M, N, s, m - 2, 2, n));
+            // D -= L2 * U2(m-2),  U2(m-2) = block (m-2, m)
+            D = bsub(D, bmul(L2, line_block(JU, M, N, n, m - 2, 2, tr, shift)));
+            store_b(F, M, N, s, m, 0, n, L2);
+        }
+        if (m >= 1) {
+            B4 B1 = line_block(JU, M, N, n, m, -1, tr, shift);
+            if (m >= 2) B1 = bsub(B1, bmul(L2, load_b(F, M, N, s, m - 2, 3, n)));
+            L1 = bmul(B1, load_b(F, M, N, s, m - 1, 2, n));
+            D = bsub(D, bmul(L1, load_b(F, M, N, s, m - 1, 3, n)));
+            store_b(F, M, N, s, m, 1, n, L1);
+        }
+        if (m + 1 < M) {
+            B4 U1 = line_block(JU, M, N, n, m, 1, tr, shift);
+            if (m >= 1) U1 = bsub(U1, bmul(L1, line_block(JU, M, N, n, m - 1, 2, tr, shift)));
+            store_b(F, M, N, s, m, 3, n, U1);
+        }
+        store_b(F, M, N, s, m, 2, n, binv(D));
+    }
+}
+
+// x = (line LU)^-1 r for every line and shift.  r and x: S consecutive complex cell fields
+// (4 planes of M x N each); x may alias r.
+__global__ void __launch_bounds__(64)
+k_line_solve(int M, int N, int S, const double* __restrict__ JU, const double* __restrict__ F,
+             int tr, const double* r, double* x) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (n >= N || s >= S) return;
+    const long C = (long)M * N, base = (long)s * 4 * C;
+    auto ld = [&](const double* a, int m, Z* y) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const long e = base + v * C + (long)n * M + m;
+            y[v] = Z{a[2 * e], a[2 * e + 1]};
+        }
+    };
+    auto st = [&](double* a, int m, const Z* y) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const long e = base + v * C + (long)n * M + m;
+            a[2 * e] = y[v].r;
+            a[2 * e + 1] = y[v].i;
+        }
+    };
+    auto mv = [&](const B4& b, const Z* y, Z* acc) {   // acc -= b y
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i] = zsub(acc[i], zmul(b.a[i][j], y[j]));
+    };
+    // forward: y_m = r_m - L2 y_{m-2} - L1 y_{m-1}  (y stored in x)
+    Z y1[4], y2[4];
+    for (int m = 0; m < M; ++m) {
+        Z y[4];
+        ld(r, m, y);
+        if (m >= 2) mv(load_b(F, M, N, s, m, 0, n), y2, y);
+        if (m >= 1) mv(load_b(F, M, N, s, m, 1, n), y1, y);
+        st(x, m, y);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            y2[v] = y1[v];
+            y1[v] = y[v];
+        }
+    }
+    // backward: x_m = Dinv (y_m - U1 x_{m+1} - U2 x_{m+2}),  U2 = block (m, m+2)
+    const Z zero = {0.0, 0.0};
+    Z x1[4], x2[4];
+    for (int m = M - 1; m >= 0; --m) {
+        Z y[4];
+        ld(x, m, y);
+        if (m + 1 < M) mv(load_b(F, M, N, s, m, 3, n), x1, y);
+        if (m + 2 < M) mv(line_block(JU, M, N, n, m, 2, tr, zero), x2, y);
+        Z xm[4] = {zero, zero, zero, zero};
+        const B4 Di = load_b(F, M, N, s, m, 2, n);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) xm[i] = zfma(Di.a[i][j], y[j], xm[i]);
+        st(x, m, xm);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            x2[v] = x1[v];
+            x1[v] = xm[v];
+        }
+    }
+}
+
+// block-Jacobi: F[(s*16 + ij)*C + cell] complex inverse of (J_cc - s I) (or its transpose)
+__global__ void __launch_bounds__(256)
+k_jacobi_factor(int C, int S, const double* __restrict__ JU, const double* __restrict__ sh, int tr,
+                double* __restrict__ F) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (c >= C || s >= S) return;
+    const Z shift = tr ? Z{sh[2 * s], -sh[2 * s + 1]} : Z{sh[2 * s], sh[2 * s + 1]};
+    const B4 Di = binv(load_jblock(JU, C, c, 0, tr, shift));
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const long e = ((long)s * 16 + 4 * i + j) * C + c;
+            F[2 * e] = Di.a[i][j].r;
+            F[2 * e + 1] = Di.a[i][j].i;
+        }
+}
+
+__global__ void __launch_bounds__(256)
+k_jacobi_solve(int C, int S, const double* __restrict__ F, const double* r, double* x) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (c >= C || s >= S) return;
+    Z y[4], o[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const long e = (long)s * 4 * C + (long)v * C + c;
+        y[v] = Z{r[2 * e], r[2 * e + 1]};
+        o[v] = Z{0.0, 0.0};
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const long e = ((long)s * 16 + 4 * i + j) * C + c;
+            o[i] = zfma(Z{F[2 * e], F[2 * e + 1]}, y[j], o[i]);
+        }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const long e = (long)s * 4 * C + (long)v * C + c;
+        x[2 * e] = o[v].r;
+        x[2 * e + 1] = o[v].i;
+    }
+}
+
+// y_k = J x_k - s_k x_k (or J^T x_k - conj(s_k) x_k) for k < S.  A workgroup holds 32 cells'
+// nine blocks in LDS and loops over the vectors; thread (cell, k-slot) handles vectors
+// k-slot, k-slot + 8, ...
+constexpr int kMvCells = 32, kMvSlots = 8;
+
+__global__ void __launch_bounds__(kMvCells * kMvSlots)
+k_block_multi(int M, int N, int S, const double* __restrict__ JU, const double* __restrict__ sh,
+              int tr, const double* __restrict__ x, double* __restrict__ y) {
+    __shared__ double sJ[9][16][kMvCells];
+    const int C = M * N;
+    const int lc = threadIdx.x % kMvCells, slot = threadIdx.x / kMvCells;
+    const int c0 = blockIdx.x * kMvCells;
+    // forward: blocks of this cell; transposed: block s of the neighbour k = c - offset(s)
+    for (int i = threadIdx.x; i < 9 * 16 * kMvCells; i += kMvCells * kMvSlots) {
+        const int cc = i % kMvCells, e = (i / kMvCells) % 16, s = i / (16 * kMvCells);
+        const int c = c0 + cc;
+        double v = 0.0;
+        if (c < C) {
+            if (!tr) {
+                v = JU[(long)(s * 16 + e) * C + c];
+            } else {
+                const int n = c / M, m = c - n * M;
+                const int dm[9] = {0, 0, 0, 0, 0, -1, -2, 1, 2};
+                const int dn[9] = {0, -1, -2, 1, 2, 0, 0, 0, 0};
+                const int mk = m - dm[s];
+                if (mk >= 0 && mk < M) {
+                    int nk = n - dn[s];
+                    nk = nk < 0 ? nk + N : (nk >= N ? nk - N : nk);
+                    // transposed block: element (i, j) of J^T block = (j, i) of J block
+                    const int ii = e / 4, jj = e % 4;
+                    v = JU[(long)(s * 16 + jj * 4 + ii) * C + nk * M + mk];
+                }
+            }
+        }
+        sJ[s][e][cc] = v;
+    }
+    __syncthreads();
+    const int c = c0 + lc;
+    if (c >= C) return;
+    const int n = c / M, m = c - n * M;
+    int nb[9];
+    {
+        auto wr = [&](int q) { return q < 0 ? q + N : (q >= N ? q - N : q); };
+        // forward: neighbour s of c;  transposed: cell k = c - offset(s)  (offset of slot s)
+        const int dm[9] = {0, 0, 0, 0, 0, -1, -2, 1, 2};
+        const int dn[9] = {0, -1, -2, 1, 2, 0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < 9; ++s) {
+            const int mm = tr ? m - dm[s] : m + dm[s];
+            const int nn = wr(tr ? n - dn[s] : n + dn[s]);
+            nb[s] = (mm >= 0 && mm < M) ? nn * M + mm : -1;
+        }
+    }
+    for (int k = slot; k < S; k += kMvSlots) {
+        const long base = (long)k * 4 * C;
+        Z acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+#pragma unroll
+        for (int s = 0; s < 9; ++s) {
+            if (nb[s] < 0) continue;
+            Z v[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const long e = base + (long)j * C + nb[s];
+                v[j] = Z{x[2 * e], x[2 * e + 1]};
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i] = zfma(Z{sJ[s][4 * i + j][lc], 0.0}, v[j], acc[i]);
+        }
+        const Z shift = tr ? Z{sh[2 * k], -sh[2 * k + 1]} : Z{sh[2 * k], sh[2 * k + 1]};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const long e = base + (long)i * C + c;
+            const Z xi = Z{x[2 * e], x[2 * e + 1]};
+            const Z r = zsub(acc[i], zmul(shift, xi));
+            y[2 * e] = r.r;
+            y[2 * e + 1] = r.i;
+        }
+    }
+}
+
+
+// Ring block of the system: D[s][4n+v][4n2+w] for ring k.  Untransposed it is the block of
+// cell (k, n) on neighbour n2 = n + o (o in -2..2, periodic); transposed, the block of cell
+// (k, n2) on n, with v and w swapped.  The shift (conj for the transposed system) is on the
+// diagonal.  One thread per complex element; elements outside the band are written as zero.
+__global__ void __launch_bounds__(256)
+k_ring_band(int M, int N, int S, const double* __restrict__ JU, const double* __restrict__ shifts,
+            int transposed, int k, double* __restrict__ D) {
+    const long n4 = 4L * N;
+    const long total = (long)S * n4 * n4;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const long col = e % n4;
+    const long row = (e / n4) % n4;
+    const int s = (int)(e / (n4 * n4));
+    const int n = (int)(row >> 2), v = (int)(row & 3);
+    const int n2 = (int)(col >> 2), w = (int)(col & 3);
+    int d = n2 - n;
+    if (d < 0) d += N;
+    // offset of n2 relative to n -> block index (c, b, bb, f, ff = 0..4)
+    int blk = -1;
+    if (d == 0) blk = 0;
+    else if (d == 1) blk = 3;
+    else if (d == 2) blk = 4;
+    else if (d == N - 1) blk = 1;
+    else if (d == N - 2) blk = 2;
+    double re = 0.0, im = 0.0;
+    if (blk >= 0) {
+        const long C = (long)M * N;
+        if (!transposed) {
+            re = JU[((long)(blk * 4 + v) * 4 + w) * C + (long)n * M + k];
+        } else {
+            // A^T[(n,v)][(n2,w)] = A[(n2,w)][(n,v)]: block of cell n2 on n = n2 - offset
+            const int tb = blk == 0 ? 0 : blk == 3 ? 1 : blk == 4 ? 2 : blk == 1 ? 3 : 4;
+            re = JU[((long)(tb * 4 + w) * 4 + v) * C + (long)n2 * M + k];
+        }
+        if (row == col) {
+            re -= shifts[2 * s];
+            im = transposed ? shifts[2 * s + 1] : -shifts[2 * s + 1];
+        }
+    }
+    D[2 * e] = re;
+    D[2 * e + 1] = im;
+}
+
+// Z[s] = beta Z[s] + alpha C Y[s] with C the coupling of ring k to ring k + dk (dk in
+// {-2, -1, 1, 2}), block diagonal in n: C[(n,v)][(n,w)].  Untransposed, C is block ii, i, o
+// or oo of ring k; transposed, the block of ring k + dk on ring k with v and w swapped.
+// Y (S, 4N, q) complex, or null for the identity (q = 4N), Z (S, 4N, q) complex.
+__global__ void __launch_bounds__(256)
+k_ring_couple(int M, int N, int S, const double* __restrict__ JU, int transposed, int k, int dk,
+              int q, double alpha, const double* __restrict__ Y, double beta,
+              double* __restrict__ Z) {
+    const long n4 = 4L * N;
+    const long total = (long)S * n4 * q;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const long col = e % q;
+    const long row = (e / q) % n4;
+    const long s = e / (n4 * q);
+    const int n = (int)(row >> 2), v = (int)(row & 3);
+    const long C = (long)M * N;
+    int blk, ring;
+    if (!transposed) {
+        blk = dk == -1 ? 5 : dk == -2 ? 6 : dk == 1 ? 7 : 8;
+        ring = k;
+    } else {
+        blk = dk == -1 ? 7 : dk == -2 ? 8 : dk == 1 ? 5 : 6;
+        ring = k + dk;
+    }
+    const double* cb = JU + (long)blk * 16 * C + (long)n * M + ring;
+    double re = 0.0, im = 0.0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        const double c = transposed ? cb[(long)(w * 4 + v) * C] : cb[(long)(v * 4 + w) * C];
+        if (Y) {
+            const long yi = (s * n4 + 4L * n + w) * q + col;
+            re = fma(c, Y[2 * yi], re);
+            im = fma(c, Y[2 * yi + 1], im);
+        } else if (col == 4L * n + w) {
+            re = c;
+        }
+    }
+    if (beta != 0.0) {
+        re = fma(alpha, re, beta * Z[2 * e]);
+        im = fma(alpha, im, beta * Z[2 * e + 1]);
+    } else {
+        re *= alpha;
+        im *= alpha;
+    }
+    Z[2 * e] = re;
+    Z[2 * e + 1] = im;
+}
+
+thread_local char g_rerr[256] = "";
+int rfail(int code, const char* msg) {
+    snprintf(g_rerr, sizeof g_rerr, "%s", msg);
+    return code;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pea_resolvent_last_error(void) { return g_rerr; }
+
+long long pea_line_factor_bytes(int M, int N, int S) {
+    return (long long)S * M * 4 * 16 * N * 16;
+}
+
+int pea_line_factor(int M, int N, int S, const double* JU, const double* shifts, int transposed,
+                    double* F, void* stream) {
+    if (M < 4 || N < 5 || S < 1 || S > 65535) return rfail(PEA_ERR_DIMS, "bad dimensions");
+    if (!JU || !shifts || !F) return rfail(PEA_ERR_NULL, "null argument");
+    k_line_factor<<<dim3((N + 63) / 64, S), 64, 0, (hipStream_t)stream>>>(M, N, S, JU, shifts,
+                                                                         transposed, F);
+    return hipGetLastError() == hipSuccess ? PEA_OK : rfail(PEA_ERR_HIP, "k_line_factor");
+}
+
+int pea_line_solve(int M, int N, int S, const double* JU, const double* F, int transposed,
+                   const double* r, double* x, void* stream) {
+    if (M < 4 || N < 5 || S < 1 || S > 65535) return rfail(PEA_ERR_DIMS, "bad dimensions");
+    if (!JU || !F || !r || !x) return rfail(PEA_ERR_NULL, "null argument");
+    k_line_solve<<<dim3((N + 63) / 64, S), 64, 0, (hipStream_t)stream>>>(M, N, S, JU, F,
+                                                                        transposed, r, x);
+    return hipGetLastError() == hipSuccess ? PEA_OK : rfail(PEA_ERR_HIP, "k_line_solve");
+}
+
+int pea_jacobi_factor(int M, int N, int S, const double* JU, const double* shifts,
+                      int transposed, double* F, void* stream) {
+    if (M < 4 || N < 5 || S < 1 || S > 65535) return rfail(PEA_ERR_DIMS, "bad dimensions");
+    if (!JU || !shifts || !F) return rfail(PEA_ERR_NULL, "null argument");
+    const int C = M * N;
+    k_jacobi_factor<<<dim3((C + 255) / 256, S), 256, 0, (hipStream_t)stream>>>(C, S, JU, shifts,
+                                                                             transposed, F);
+    return hipGetLastError() == hipSuccess ? PEA_OK : rfail(PEA_ERR_HIP, "k_jacobi_factor");
+}
+
+int pea_jacobi_solve(int M, int N, int S, const double* F, const double* r, double* x,
+                     void* stream) {
+    if (M < 4 || N < 5 || S < 1 || S > 65535) return rfail(PEA_ERR_DIMS, "bad dimensions");
+    if (!F || !r || !x) return rfail(PEA_ERR_NULL, "null argument");
+    const int C = M * N;
+    k_jacobi_solve<<<dim3((C + 255) / 256, S), 256, 0, (hipStream_t)stream>>>(C, S, F, r, x);
+    return hipGetLastError() == hipSuccess ? PEA_OK : rfail(PEA_ERR_HIP, "k_jacobi_solve");
+}
+
+int pea_block_multi(int M, int N, int S, const double* JU, const double* shifts, int transposed,
+                    const double* x, double* y, void* stream) {
+    if (M < 4 || N < 5 || S < 1) return rfail(PEA_ERR_DIMS, "bad dimensions");
+    if (!JU || !shifts || !x || !y) return rfail(PEA_ERR_NULL, "null argument");
+    const int C = M * N;
+    k_block_multi<<<(C + kMvCells - 1) / kMvCells, kMvCells * kMvSlots, 0, (hipStream_t)stream>>>(
+        M, N, S, JU, shifts, transposed, x, y);
+    return hipGetLastError() == hipSuccess ? PEA_OK : rfail(PEA_ERR_HIP, "k_block_multi");
+}
+
+int pea_ring_band(int M, int N, int S, const double* JU, const double* shifts, int transposed,
+                  int k, double* D, void* stream) {
+    if (M < 4 || N < 5 || S < 1) return rfail(PEA_ERR_DIMS, "bad dimensions");
+    if (k < 0 || k >= M) return rfail(PEA_ERR_DIMS, "ring index out of range");
+    if (!JU || !shifts || !D) return rfail(PEA_ERR_NULL, "null argument");
+    const long total = (long)S * 16L * N * N;
+    const long blocks = (total + 255) / 256;
+    if (blocks > 0x7fffffffL) return rfail(PEA_ERR_DIMS, "ring block too large");
+    k_ring_band<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(M, N, S, JU, shifts,
+                                                                   transposed, k, D);
+    return hipGetLastError() == hipSuccess ? PEA_OK : rfail(PEA_ERR_HIP, "k_ring_band");
+}
+
+int pea_ring_couple(int M, int N, int S, const double* JU, int transposed, int k, int dk, int q,
+                    double alpha, const double* Y, double beta, double* Z, void* stream) {
+    if (M < 4 || N < 5 || S < 1 || q < 1) return rfail(PEA_ERR_DIMS, "bad dimensions");
+    if (!(dk == -2 || dk == -1 || dk == 1 || dk == 2))
+        return rfail(PEA_ERR_DIMS, "ring offset must be -2, -1, 1 or 2");
+    if (k < 0 || k >= M || k + dk < 0 || k + dk >= M)
+        return rfail(PEA_ERR_DIMS, "ring index out of range");
+    if (!Y && q != 4 * N) return rfail(PEA_ERR_DIMS, "identity operand needs q = 4N");
+    if (!JU || !Z) return rfail(PEA_ERR_NULL, "null argument");
+    const long total = (long)S * 4L * N * q;
+    const long blocks = (total + 255) / 256;
+    if (blocks > 0x7fffffffL) return rfail(PEA_ERR_DIMS, "operand too large");
+    k_ring_couple<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(M, N, S, JU, transposed, k,
+                                                                     dk, q, alpha, Y, beta, Z);
+    return hipGetLastError() == hipSuccess ? PEA_OK : rfail(PEA_ERR_HIP, "k_ring_couple");
+}
+
+}  // extern "C"

@@ -33,19 +33,28 @@ PEA_HD double sign1(double x) { return copysign(1.0, x); }
 struct Cx {
     double r, i;
 };
-PEA_HD Cx operator+(Cx a, Cx b) { return {a.r + b.r, a.i + b.i}; }
-PEA_HD Cx operator-(Cx a, Cx b) { return {a.r - b.r, a.i - b.i}; }
+// Complex-step arithmetic is never contracted into FMAs: the reference evaluates R in complex
+// arithmetic with separately rounded products and sums, and at kinks of abs/max (e.g. the
+// pressure sensor of a wall face at the free stream, where p_ll - 2 p_l + p_r is round-off)
+// the sign of a round-off-sized real part selects the derivative branch.  Matching the
+// reference's rounding keeps its Jacobians, and so its Newton iterates, reproducible.
+#define PEA_NOFMA _Pragma("clang fp contract(off)")
+PEA_HD Cx operator+(Cx a, Cx b) { PEA_NOFMA return {a.r + b.r, a.i + b.i}; }
+PEA_HD Cx operator-(Cx a, Cx b) { PEA_NOFMA return {a.r - b.r, a.i - b.i}; }
 PEA_HD Cx operator-(Cx a) { return {-a.r, -a.i}; }
-PEA_HD Cx operator*(Cx a, Cx b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
-PEA_HD Cx operator*(double s, Cx a) { return {s * a.r, s * a.i}; }
-PEA_HD Cx operator*(Cx a, double s) { return {a.r * s, a.i * s}; }
-PEA_HD Cx operator/(Cx a, double s) { return {a.r / s, a.i / s}; }
-PEA_HD Cx operator+(Cx a, double s) { return {a.r + s, a.i}; }
-PEA_HD Cx operator-(Cx a, double s) { return {a.r - s, a.i}; }
-PEA_HD Cx operator-(double s, Cx a) { return {s - a.r, -a.i}; }
-PEA_HD Cx operator+(double s, Cx a) { return {s + a.r, a.i}; }
+PEA_HD Cx operator*(Cx a, Cx b) {
+    PEA_NOFMA return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r};
+}
+PEA_HD Cx operator*(double s, Cx a) { PEA_NOFMA return {s * a.r, s * a.i}; }
+PEA_HD Cx operator*(Cx a, double s) { PEA_NOFMA return {a.r * s, a.i * s}; }
+PEA_HD Cx operator/(Cx a, double s) { PEA_NOFMA return {a.r / s, a.i / s}; }
+PEA_HD Cx operator+(Cx a, double s) { PEA_NOFMA return {a.r + s, a.i}; }
+PEA_HD Cx operator-(Cx a, double s) { PEA_NOFMA return {a.r - s, a.i}; }
+PEA_HD Cx operator-(double s, Cx a) { PEA_NOFMA return {s - a.r, -a.i}; }
+PEA_HD Cx operator+(double s, Cx a) { PEA_NOFMA return {s + a.r, a.i}; }
 // Smith's scaled division (the range reduction Fortran compilers use for complex division).
 PEA_HD Cx operator/(Cx a, Cx b) {
+    PEA_NOFMA
     if (fabs(b.r) >= fabs(b.i)) {
         double t = b.i / b.r, d = b.r + b.i * t;
         return {(a.r + a.i * t) / d, (a.i - a.r * t) / d};
@@ -58,6 +67,7 @@ PEA_HD double val(Cx x) { return x.r; }
 PEA_HD Cx sabs(Cx x) { return {fabs(x.r), x.i * copysign(1.0, x.r)}; }
 // principal square root, computed without cancellation
 PEA_HD Cx ssqrt(Cx z) {
+    PEA_NOFMA
     if (z.r == 0.0 && z.i == 0.0) return {0.0, z.i};
     double t = sqrt(0.5 * (fabs(z.r) + hypot(z.r, z.i)));
     if (z.r >= 0.0) return {t, z.i / (2.0 * t)};

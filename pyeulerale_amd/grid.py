@@ -7,6 +7,7 @@ and ``n`` the angular vertex index; the closing column ``n = N`` duplicates ``n 
 
 from __future__ import annotations
 
+import gzip
 from pathlib import Path
 
 import numpy as np
@@ -101,9 +102,11 @@ def read_plot3d(path: str | Path) -> np.ndarray:
     """Reads a single-block 2-D PLOT3D grid as the reference does (core.py:213-227).
 
     Format: block count, then ``Mpts Npts``, then all x followed by all z with the angular index
-    fastest.  Returns float64 vertices of shape ``(2, Mpts, Npts)``.
+    fastest.  Returns float64 vertices of shape ``(2, Mpts, Npts)``.  A ``.gz`` file is read
+    through gzip.
     """
-    with open(path) as fh:
+    opener = gzip.open if str(path).endswith(".gz") else open
+    with opener(path, "rt") as fh:
         fh.readline()
         mpts, npts = (int(t) for t in fh.readline().split()[:2])
         vals = np.array(fh.read().split(), dtype=float)

@@ -85,6 +85,10 @@ class Grid:
         self.M, self.N = int(M), int(N)
 
     @property
+    def C(self):
+        return self.M * self.N
+
+    @property
     def cell_shape(self):
         return (4, self.N, self.M)
 

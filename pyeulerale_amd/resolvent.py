@@ -1,15 +1,12 @@
-"""Resolvent (dR/dU - s I)^-1 of the linearized discretization.
+"""The reference's resolvent method, kept for comparison: SuperLU on the exported block matrix.
 
-Two solvers:
-
-* :func:`direct_solve` -- the reference's method (core.py:512-599): the dR/dU blocks are
-  assembled by complex step on the GPU, exported in O(nnz) to BSR with the shift on the
-  diagonal (the reference's ``convert_odes_wrt_states`` scans all cell pairs, f90:765-814) and
-  factorised by SuperLU (``scipy.sparse.linalg.spsolve``, core.py:562, 596-598).  Exact; meant
-  for grids whose LU fits in host memory.
-* :func:`gmres_solve` -- matrix-free restarted GMRES on the GPU for many shifts at once, with
-  the hand-written tangent (or adjoint) kernel as the operator and radial-line block
-  preconditioning (see :mod:`pyeulerale_amd.precond`).
+:func:`direct_solve` follows core.py:512-599.  The dR/dU blocks are assembled by complex step
+on the GPU, exported in O(nnz) to BSR with the shift on the diagonal (the reference's
+``convert_odes_wrt_states`` scans all cell pairs, f90:765-814), and factorised by SuperLU on
+the host (``scipy.sparse.linalg.spsolve``, core.py:562, 596-598).  Exact, but host-bound: about
+16 s per shift at 129x129 vertices.  ``solver="superlu"`` of :class:`SpatialDiscretization`
+selects it.  The GPU solvers are :mod:`pyeulerale_amd.ringlu` (exact) and
+:mod:`pyeulerale_amd.gmres` (iterative).
 """
 
 from __future__ import annotations
@@ -41,8 +38,3 @@ def direct_solve(sd, b: torch.Tensor, shift: complex, adjoint: bool) -> torch.Te
     sol = spsolve(A, rhs)
     x = torch.as_tensor(np.asarray(sol, dtype=complex)).reshape(g.N, g.M, 4).permute(2, 0, 1)
     return x.contiguous().to(b.device)
-
-
-def gmres_solve(sd, b: torch.Tensor, shifts, adjoint: bool = False, **opts) -> torch.Tensor:
-    from .gmres import BatchedGMRES
-    return BatchedGMRES.from_discretization(sd, shifts, adjoint=adjoint, **opts).solve(b)
