@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-large", action="store_true", help="skip the 1025x513 products")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-freq", action="store_true", help="skip the frequency-response solve")
+    ap.add_argument("--no-superlu", action="store_true",
+                    help="skip timing the reference's host SuperLU solve")
     return ap.parse_args()
 
 
@@ -81,6 +84,10 @@ def graph_time_ms(torch, fn, reps=20):
     b.record()
     torch.cuda.synchronize()
     return a.elapsed_time(b) / (3 * reps)
+
+
+def progress(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def rel(a, b):
@@ -135,6 +142,51 @@ def products(torch, K, A, Rv, thickness, mach, aoa, alpha, alpha_dot, far, reps)
         "jvp_hbm_frac": (160 * C + 32 * P) / (t["jvp"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "vjp_hbm_frac": (160 * C + 32 * P) / (t["vjp"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
     }
+    return res
+
+
+def freq_response(torch, superlu: bool, shifts: int = 32):
+    """Steady state and 32-frequency pitch response on the reference's own 129x129 NACA0004 grid
+    (examples/response/run.py, Ma 0.5, chord 10): all shifts in one batched ring-LU solve
+    against one shift at a time, plus the reference's host SuperLU solve of one shift."""
+    from pyeulerale_amd.core import SpatialDiscretization
+    from pyeulerale_amd.drivers import frequency_response, pseudo_transient_continuation
+    grid = ROOT / "tests" / "golden" / "examples" / "naca0004_129x129.plot3d.gz"
+    sd = SpatialDiscretization(grid_file=grid, mach_number=0.5, angle_of_attack=0.0,
+                               coefficient_length=10.0, coefficient_center=(0.0, 0.0))
+    sync = torch.cuda.synchronize
+    t0 = time.perf_counter()
+    hist = pseudo_transient_continuation(sd, rtol=1e-9, max_iter=100, log=None)
+    sync()
+    t_ptc = time.perf_counter() - t0
+    k = np.logspace(-2, 1, shifts)
+    frequency_response(sd, k[:2], chord=10.0, axis_location=0.0, log=None)   # library warm-up
+    sync()
+    t0 = time.perf_counter()
+    _, cl_b, cm_b = frequency_response(sd, k, chord=10.0, axis_location=0.0, log=None)
+    sync()
+    t_b = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, cl_s, cm_s = frequency_response(sd, k, chord=10.0, axis_location=0.0, log=None,
+                                       batched=False)
+    sync()
+    t_s = time.perf_counter() - t0
+    res = {
+        "grid": "naca0004_129x129 (reference examples/grids), M=128, N=128", "shifts": shifts,
+        "reduced_frequencies": "logspace(-2, 1, 32)",
+        "ptc_iterations": len(hist), "ptc_final_rel_residual": hist[-1][2], "ptc_s": t_ptc,
+        "batched_s": t_b, "sequential_s": t_s, "speedup_batched_vs_sequential": t_s / t_b,
+        "max_rel_diff_batched_vs_sequential": float(max(
+            np.max(np.abs(cl_b - cl_s) / np.abs(cl_s)), np.max(np.abs(cm_b - cm_s) / np.abs(cm_s)))),
+        "cl_k0.01": [float(cl_b[0].real), float(cl_b[0].imag)],
+    }
+    if superlu:
+        rhs = sd.apply_odes_wrt_vertices_fwd(sd.vertices.new_zeros(sd.vertices.shape,
+                                                                   dtype=torch.complex128) + 1.0)
+        t0 = time.perf_counter()
+        sd.solve_odes_wrt_states_fwd(rhs, shift=0.3j, solver="superlu")
+        res["superlu_one_shift_s"] = time.perf_counter() - t0
+        res["speedup_batched_per_shift_vs_superlu"] = res["superlu_one_shift_s"] / (t_b / shifts)
     return res
 
 
@@ -196,11 +248,17 @@ def main():
     achieved = bytes_launch / (kern_ms * 1e-3) / 1e9
     value = BATCH * args.steps / wall
 
+    progress(f"residual batch: {kern_ms:.4f} ms per launch")
     extra = {"products_257x129": products(torch, K, 257, 129, 0.04, mach, aoa, alpha,
                                           alpha_dot, 20.0, 50)}
+    progress("products 257x129 done")
     if not args.no_large:
         extra["products_1025x513"] = products(torch, K, 1025, 513, 0.04, 0.5, 0.0, 0.0, 0.0,
                                               30.0, 10)
+        progress("products 1025x513 done")
+    if not args.no_freq:
+        extra["frequency_response_129x129"] = freq_response(torch, not args.no_superlu)
+        progress("frequency response done")
     line = {
         "metric": "residual evals/s + hand-written VJP vs complex-step JVP speedup (FP64, 1 GPU)",
         "value": value, "unit": "residual evals/s", "n_gpus": 1, "steps": args.steps,

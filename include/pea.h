@@ -198,17 +198,25 @@ int pea_line_solve(int M, int N, int S, const double *JU, const double *F, int t
 /* Exact direct resolvent by block LU over radial rings (replaces the SuperLU factorisation of
  * core.py:562-564 and 596-598).  Ring k holds every cell at radial index k; its 4N unknowns
  * are ordered (n, v).  The system is block pentadiagonal over rings: a dense ring block and
- * couplings to rings k-2..k+2 that are block diagonal in n.
- * pea_ring_band: D[s] (S x 4N x 4N complex, row-major) = ring block k of the system, the
- *   shift on the diagonal, zero outside the periodic band.
+ * couplings to rings k-2..k+2 that are block diagonal in n.  Dense S x 4N x q complex
+ * matrices are row-major (colmajor = 0) or column-major (colmajor = 1) per batch entry.
+ * pea_ring_band: D[s] (4N x 4N) = ring block k of the system, the shift on the diagonal,
+ *   zero outside the periodic band.
  * pea_ring_couple: Z[s] = beta Z[s] + alpha C Y[s], C = coupling of ring k to ring k + dk
- *   (dk in -2, -1, 1, 2); Y, Z are S x 4N x q complex; Y = NULL stands for the identity
- *   (q = 4N), which writes C as a dense matrix.
- * The eliminations between these calls are dense complex GEMM and LU. */
+ *   (dk in -2, -1, 1, 2); Y = NULL stands for the identity (q = 4N), which writes C as a
+ *   dense matrix.
+ * pea_dense_lu_factor / pea_dense_lu_solve: batched partial-pivoting LU of S column-major
+ *   n x n complex matrices (rocSOLVER getrf/getrs, strided batched); ipiv S*n and info S
+ *   int32 on the device; B (n x nrhs per entry, column-major) is overwritten by the solution.
+ * The eliminations between these calls are dense complex GEMMs. */
 int pea_ring_band(int M, int N, int S, const double *JU, const double *shifts, int transposed,
-                  int k, double *D, void *stream);
-int pea_ring_couple(int M, int N, int S, const double *JU, int transposed, int k, int dk, int q,
-                    double alpha, const double *Y, double beta, double *Z, void *stream);
+                  int k, int colmajor, double *D, void *stream);
+int pea_ring_couple(int M, int N, int S, const double *JU, int transposed, int k, int dk,
+                    int colmajor, int q, double alpha, const double *Y, double beta, double *Z,
+                    void *stream);
+int pea_dense_lu_factor(int S, int n, double *A, int *ipiv, int *info, void *stream);
+int pea_dense_lu_solve(int S, int n, int nrhs, const double *A, const int *ipiv, double *B,
+                       void *stream);
 
 /* Block-Jacobi preconditioner: inverses of the 4x4 diagonal blocks (J_cc - s I);
  * F holds S*16*M*N complex values. */

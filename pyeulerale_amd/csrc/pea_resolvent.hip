@@ -19,7 +19,11 @@
 //   matrix when Y is null.  The dense eliminations between them are library GEMM/LU calls.
 #include <hip/hip_runtime.h>
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include <cstdio>
+#include <mutex>
 
 #include "../../include/pea.h"
 
@@ -401,19 +405,21 @@ k_block_multi(int M, int N, int S, const double* __restrict__ JU, const double* 
 }
 
 
-// Ring block of the system: D[s][4n+v][4n2+w] for ring k.  Untransposed it is the block of
+// Ring block of the system: D[s][4n+v][4n2+w] for ring k, row-major or column-major.  Untransposed it is the block of
 // cell (k, n) on neighbour n2 = n + o (o in -2..2, periodic); transposed, the block of cell
 // (k, n2) on n, with v and w swapped.  The shift (conj for the transposed system) is on the
 // diagonal.  One thread per complex element; elements outside the band are written as zero.
 __global__ void __launch_bounds__(256)
 k_ring_band(int M, int N, int S, const double* __restrict__ JU, const double* __restrict__ shifts,
-            int transposed, int k, double* __restrict__ D) {
+            int transposed, int k, int colmajor, double* __restrict__ D) {
     const long n4 = 4L * N;
     const long total = (long)S * n4 * n4;
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= total) return;
-    const long col = e % n4;
-    const long row = (e / n4) % n4;
+    // consecutive threads write consecutive addresses in either layout
+    const long fast = e % n4, slow = (e / n4) % n4;
+    const long col = colmajor ? slow : fast;
+    const long row = colmajor ? fast : slow;
     const int s = (int)(e / (n4 * n4));
     const int n = (int)(row >> 2), v = (int)(row & 3);
     const int n2 = (int)(col >> 2), w = (int)(col & 3);
@@ -448,18 +454,20 @@ k_ring_band(int M, int N, int S, const double* __restrict__ JU, const double* __
 // Z[s] = beta Z[s] + alpha C Y[s] with C the coupling of ring k to ring k + dk (dk in
 // {-2, -1, 1, 2}), block diagonal in n: C[(n,v)][(n,w)].  Untransposed, C is block ii, i, o
 // or oo of ring k; transposed, the block of ring k + dk on ring k with v and w swapped.
-// Y (S, 4N, q) complex, or null for the identity (q = 4N), Z (S, 4N, q) complex.
+// Y (S, 4N, q) complex, or null for the identity (q = 4N), Z (S, 4N, q) complex; both
+// row-major or both column-major (each batch entry a 4N x q matrix).
 __global__ void __launch_bounds__(256)
 k_ring_couple(int M, int N, int S, const double* __restrict__ JU, int transposed, int k, int dk,
-              int q, double alpha, const double* __restrict__ Y, double beta,
+              int colmajor, int q, double alpha, const double* __restrict__ Y, double beta,
               double* __restrict__ Z) {
     const long n4 = 4L * N;
     const long total = (long)S * n4 * q;
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= total) return;
-    const long col = e % q;
-    const long row = (e / q) % n4;
     const long s = e / (n4 * q);
+    const long in = e - s * n4 * q;
+    const long col = colmajor ? in / n4 : in % q;
+    const long row = colmajor ? in % n4 : in / q;
     const int n = (int)(row >> 2), v = (int)(row & 3);
     const long C = (long)M * N;
     int blk, ring;
@@ -476,7 +484,8 @@ k_ring_couple(int M, int N, int S, const double* __restrict__ JU, int transposed
     for (int w = 0; w < 4; ++w) {
         const double c = transposed ? cb[(long)(w * 4 + v) * C] : cb[(long)(v * 4 + w) * C];
         if (Y) {
-            const long yi = (s * n4 + 4L * n + w) * q + col;
+            const long yr = 4L * n + w;
+            const long yi = s * n4 * q + (colmajor ? col * n4 + yr : yr * q + col);
             re = fma(c, Y[2 * yi], re);
             im = fma(c, Y[2 * yi + 1], im);
         } else if (col == 4L * n + w) {
@@ -492,6 +501,32 @@ k_ring_couple(int M, int N, int S, const double* __restrict__ JU, int transposed
     }
     Z[2 * e] = re;
     Z[2 * e + 1] = im;
+}
+
+
+// One rocBLAS handle per device for the dense factorisations, created on first use.
+constexpr int kMaxDevices = 64;
+rocblas_handle g_handles[kMaxDevices] = {};
+std::mutex g_handle_mutex;
+
+rocblas_handle dense_handle(hipStream_t stream, int* err) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        *err = 1;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_handle_mutex);
+    if (!g_handles[dev] && rocblas_create_handle(&g_handles[dev]) != rocblas_status_success) {
+        g_handles[dev] = nullptr;
+        *err = 1;
+        return nullptr;
+    }
+    if (rocblas_set_stream(g_handles[dev], stream) != rocblas_status_success) {
+        *err = 1;
+        return nullptr;
+    }
+    *err = 0;
+    return g_handles[dev];
 }
 
 thread_local char g_rerr[256] = "";
@@ -558,7 +593,7 @@ int pea_block_multi(int M, int N, int S, const double* JU, const double* shifts,
 }
 
 int pea_ring_band(int M, int N, int S, const double* JU, const double* shifts, int transposed,
-                  int k, double* D, void* stream) {
+                  int k, int colmajor, double* D, void* stream) {
     if (M < 4 || N < 5 || S < 1) return rfail(PEA_ERR_DIMS, "bad dimensions");
     if (k < 0 || k >= M) return rfail(PEA_ERR_DIMS, "ring index out of range");
     if (!JU || !shifts || !D) return rfail(PEA_ERR_NULL, "null argument");
@@ -566,12 +601,13 @@ int pea_ring_band(int M, int N, int S, const double* JU, const double* shifts, i
     const long blocks = (total + 255) / 256;
     if (blocks > 0x7fffffffL) return rfail(PEA_ERR_DIMS, "ring block too large");
     k_ring_band<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(M, N, S, JU, shifts,
-                                                                   transposed, k, D);
+                                                                   transposed, k, colmajor, D);
     return hipGetLastError() == hipSuccess ? PEA_OK : rfail(PEA_ERR_HIP, "k_ring_band");
 }
 
-int pea_ring_couple(int M, int N, int S, const double* JU, int transposed, int k, int dk, int q,
-                    double alpha, const double* Y, double beta, double* Z, void* stream) {
+int pea_ring_couple(int M, int N, int S, const double* JU, int transposed, int k, int dk,
+                    int colmajor, int q, double alpha, const double* Y, double beta, double* Z,
+                    void* stream) {
     if (M < 4 || N < 5 || S < 1 || q < 1) return rfail(PEA_ERR_DIMS, "bad dimensions");
     if (!(dk == -2 || dk == -1 || dk == 1 || dk == 2))
         return rfail(PEA_ERR_DIMS, "ring offset must be -2, -1, 1 or 2");
@@ -582,9 +618,36 @@ int pea_ring_couple(int M, int N, int S, const double* JU, int transposed, int k
     const long total = (long)S * 4L * N * q;
     const long blocks = (total + 255) / 256;
     if (blocks > 0x7fffffffL) return rfail(PEA_ERR_DIMS, "operand too large");
-    k_ring_couple<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(M, N, S, JU, transposed, k,
-                                                                     dk, q, alpha, Y, beta, Z);
+    k_ring_couple<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(
+        M, N, S, JU, transposed, k, dk, colmajor, q, alpha, Y, beta, Z);
     return hipGetLastError() == hipSuccess ? PEA_OK : rfail(PEA_ERR_HIP, "k_ring_couple");
+}
+
+int pea_dense_lu_factor(int S, int n, double* A, int* ipiv, int* info, void* stream) {
+    if (S < 1 || n < 1) return rfail(PEA_ERR_DIMS, "bad dimensions");
+    if (!A || !ipiv || !info) return rfail(PEA_ERR_NULL, "null argument");
+    int err = 0;
+    rocblas_handle h = dense_handle((hipStream_t)stream, &err);
+    if (err) return rfail(PEA_ERR_HIP, "rocBLAS handle");
+    const rocblas_status st = rocsolver_zgetrf_strided_batched(
+        h, n, n, reinterpret_cast<rocblas_double_complex*>(A), n, (rocblas_stride)n * n, ipiv,
+        (rocblas_stride)n, info, S);
+    return st == rocblas_status_success ? PEA_OK : rfail(PEA_ERR_HIP, "rocsolver getrf");
+}
+
+int pea_dense_lu_solve(int S, int n, int nrhs, const double* A, const int* ipiv, double* B,
+                       void* stream) {
+    if (S < 1 || n < 1 || nrhs < 1) return rfail(PEA_ERR_DIMS, "bad dimensions");
+    if (!A || !ipiv || !B) return rfail(PEA_ERR_NULL, "null argument");
+    int err = 0;
+    rocblas_handle h = dense_handle((hipStream_t)stream, &err);
+    if (err) return rfail(PEA_ERR_HIP, "rocBLAS handle");
+    const rocblas_status st = rocsolver_zgetrs_strided_batched(
+        h, rocblas_operation_none, n, nrhs,
+        reinterpret_cast<rocblas_double_complex*>(const_cast<double*>(A)), n,
+        (rocblas_stride)n * n, ipiv, (rocblas_stride)n,
+        reinterpret_cast<rocblas_double_complex*>(B), n, (rocblas_stride)n * nrhs, S);
+    return st == rocblas_status_success ? PEA_OK : rfail(PEA_ERR_HIP, "rocsolver getrs");
 }
 
 }  // extern "C"

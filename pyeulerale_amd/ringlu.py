@@ -17,10 +17,12 @@ dense ring block, needs per ring k::
 
 and the solve is y_k = P_k^-1 (b_k - B_k y_{k-1} - A[k,k-2] y_{k-2}) followed by
 x_k = y_k - W1[k] x_{k+1} - W2[k] x_{k+2}.  The dense products and LU factorisations are batched
-over the shifts (complex128 GEMM/getrf/getrs); the ring blocks and couplings come from the
-assembled complex-step blocks through ``pea_ring_band`` / ``pea_ring_couple``.  Memory per
-shift is 3 dense (4N)^2 complex blocks per ring (LU of P_k, W1, W2): 2.6 GB per shift at
-129x129 vertices, 10.7 GB at 257x129.  One step of iterative refinement with the exact block
+over the shifts (complex128 GEMMs, and rocSOLVER's strided-batched getrf/getrs through
+``pea_dense_lu_factor`` / ``pea_dense_lu_solve``, on column-major storage); the ring blocks and
+couplings come from the assembled complex-step blocks through ``pea_ring_band`` /
+``pea_ring_couple``.  Memory per
+shift is 3 dense (4N)^2 complex blocks per ring (LU of P_k, W1, W2): 1.6 GB per shift at
+129x129 vertices, 6.4 GB at 257x129 (and 412 GB at 1025x513, beyond one GPU).  One step of iterative refinement with the exact block
 operator (``pea_block_multi``) follows each solve.
 
 The adjoint system (dR/dU)^T - conj(s) I (core.py:594-598) is factorised the same way with the
@@ -68,18 +70,19 @@ class RingLU:
         self._factor()
 
     # ------------------------------------------------------------------ kernels
-    def _couple(self, k, dk, Y, alpha=1.0, Z=None, beta=0.0):
-        """Z = beta Z + alpha C_{k,k+dk} Y (Y None: the coupling itself as a dense matrix)."""
+    # Dense matrices live in column-major storage (the layout of the rocSOLVER factorisation):
+    # a storage tensor T of shape (S, q, n) holds the logical (S, n, q) matrix T.mT.
+    def _couple(self, k, dk, Y=None, alpha=1.0, Z=None, beta=0.0):
+        """Z = beta Z + alpha C_{k,k+dk} Y on column-major storage (Y None: C itself, dense)."""
         g, S, n = self.g, self.S, 4 * self.g.N
-        q = n if Y is None else Y.shape[-1]
-        if Y is not None:
-            Y = Y.contiguous()          # lu_solve may return column-major batches
-        if Z is not None and not Z.is_contiguous():
-            raise RuntimeError("ring coupling output must be contiguous")
+        q = n if Y is None else Y.shape[1]
         if Z is None:
-            Z = torch.empty((S, n, q), dtype=C128, device=self.device)
+            Z = torch.empty((S, q, n), dtype=C128, device=self.device)
             beta = 0.0
-        _lib.call("pea_ring_couple", g.M, g.N, S, _p(self.JU), int(self.adjoint), k, dk, q,
+        for t in (Y, Z):
+            if t is not None and not t.is_contiguous():
+                raise RuntimeError("ring operands must be contiguous storage")
+        _lib.call("pea_ring_couple", g.M, g.N, S, _p(self.JU), int(self.adjoint), k, dk, 1, q,
                   float(alpha), _p(Y), float(beta), _p(Z), K._stream())
         return Z
 
@@ -87,43 +90,49 @@ class RingLU:
         g, S, n = self.g, self.S, 4 * self.g.N
         D = torch.empty((S, n, n), dtype=C128, device=self.device)
         _lib.call("pea_ring_band", g.M, g.N, S, _p(self.JU), _p(self.sh), int(self.adjoint), k,
-                  _p(D), K._stream())
+                  1, _p(D), K._stream())
         return D
+
+    def _lu_solve(self, k, B):
+        """B <- P_k^-1 B in place; B column-major storage (S, q, n)."""
+        n = 4 * self.g.N
+        _lib.call("pea_dense_lu_solve", self.S, n, B.shape[1], _p(self.LU[k]), _p(self.piv[k]),
+                  _p(B), K._stream())
+        return B
 
     # ------------------------------------------------------------------ factorisation
     def _factor(self):
-        M = self.g.M
+        M, S, n = self.g.M, self.S, 4 * self.g.N
         self.LU, self.piv, self.W1, self.W2 = [None] * M, [None] * M, [None] * M, [None] * M
-        infos = []
+        info = torch.zeros((M, S), dtype=torch.int32, device=self.device)
         for k in range(M):
             P = self._band(k)
-            R1 = self._couple(k, 1, None) if k + 1 < M else None
+            R1 = self._couple(k, 1) if k + 1 < M else None
             if k >= 1:
-                B = self._couple(k, -1, None)
+                B = self._couple(k, -1)
                 if k >= 2:
                     self._couple(k, -2, self.W1[k - 2], -1.0, B, 1.0)
-                P.baddbmm_(B, self.W1[k - 1], alpha=-1)
+                P.mT.baddbmm_(B.mT, self.W1[k - 1].mT, alpha=-1)
                 if k >= 2:
                     self._couple(k, -2, self.W2[k - 2], -1.0, P, 1.0)
                 if R1 is not None:
-                    R1.baddbmm_(B, self.W2[k - 1], alpha=-1)
+                    R1.mT.baddbmm_(B.mT, self.W2[k - 1].mT, alpha=-1)
                 del B
-            LU, piv, info = torch.linalg.lu_factor_ex(P)
-            del P
-            infos.append(info)
-            self.LU[k], self.piv[k] = LU, piv
+            piv = torch.empty((S, n), dtype=torch.int32, device=self.device)
+            _lib.call("pea_dense_lu_factor", S, n, _p(P), _p(piv), _p(info[k]), K._stream())
+            self.LU[k], self.piv[k] = P, piv
             if R1 is not None:
-                self.W1[k] = torch.linalg.lu_solve(LU, piv, R1)
-                del R1
+                self.W1[k] = self._lu_solve(k, R1)
             if k + 2 < M:
-                self.W2[k] = torch.linalg.lu_solve(LU, piv, self._couple(k, 2, None))
-        bad = torch.stack(infos).ne(0).any(0).cpu().numpy()
+                self.W2[k] = self._lu_solve(k, self._couple(k, 2))
+        bad = info.ne(0).any(0).cpu().numpy()
         if bad.any():
             raise _lib.PeaError(f"singular ring block for shifts {self.shifts[bad]}")
 
     # ------------------------------------------------------------------ solve
     def _sweep(self, b):
-        """Forward and back substitution; b of shape (S, M, 4N, 1) ring-major."""
+        """Forward and back substitution; b of shape (S, M, 1, 4N): per ring the storage of a
+        column vector."""
         M = self.g.M
         y = [None] * M
         for k in range(M):
@@ -131,23 +140,24 @@ class RingLU:
             if k >= 1:
                 self._couple(k, -1, y[k - 1], -1.0, r, 1.0)
                 if k >= 2:
-                    self._couple(k, -2, torch.bmm(self.W1[k - 2], y[k - 1]), 1.0, r, 1.0)
+                    t = torch.bmm(self.W1[k - 2].mT, y[k - 1].mT).mT.contiguous()
+                    self._couple(k, -2, t, 1.0, r, 1.0)
                     self._couple(k, -2, y[k - 2], -1.0, r, 1.0)
-            y[k] = torch.linalg.lu_solve(self.LU[k], self.piv[k], r)
+            y[k] = self._lu_solve(k, r)
         x = [None] * M
         for k in range(M - 1, -1, -1):
-            xk = y[k]
+            xk = y[k].mT
             if k + 1 < M:
-                xk = torch.baddbmm(xk, self.W1[k], x[k + 1], alpha=-1)
+                xk = torch.baddbmm(xk, self.W1[k].mT, x[k + 1].mT, alpha=-1)
             if k + 2 < M:
-                xk = xk.baddbmm_(self.W2[k], x[k + 2], alpha=-1)
-            x[k] = xk
+                xk = xk.baddbmm_(self.W2[k].mT, x[k + 2].mT, alpha=-1)
+            x[k] = xk.mT
         return torch.stack(x, 1)
 
     def _to_rings(self, v):
-        """(S, 4, N, M) canonical storage -> (S, M, 4N, 1) with ring unknowns ordered (n, v)."""
+        """(S, 4, N, M) canonical storage -> (S, M, 1, 4N), ring unknowns ordered (n, v)."""
         S, g = self.S, self.g
-        return v.permute(0, 3, 2, 1).reshape(S, g.M, 4 * g.N, 1).contiguous()
+        return v.permute(0, 3, 2, 1).reshape(S, g.M, 1, 4 * g.N).contiguous()
 
     def _from_rings(self, r):
         S, g = self.S, self.g
