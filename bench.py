@@ -44,6 +44,10 @@ def parse():
     ap.add_argument("--no-freq", action="store_true", help="skip the frequency-response solve")
     ap.add_argument("--no-superlu", action="store_true",
                     help="skip timing the reference's host SuperLU solve")
+    ap.add_argument("--profile", action="store_true",
+                    help="plain launches only (no graphs) for counter collection: a 512 MiB "
+                         "copy for calibration, the batched residual, and the JVP / VJP / "
+                         "complex-step products at 1025x513, 5 launches each")
     return ap.parse_args()
 
 
@@ -190,6 +194,42 @@ def freq_response(torch, superlu: bool, shifts: int = 32):
     return res
 
 
+def profile_launches(torch, K):
+    """Kernel launches for hardware-counter passes (rocprofv3 --pmc): every dispatch is a plain
+    launch on torch's current stream.  A device-to-device copy of 512 MiB comes first; its
+    FETCH_SIZE / WRITE_SIZE give the calibration of the byte counters."""
+    src = torch.empty(64 * 2 ** 20, dtype=torch.float64, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    del src, dst
+    mach, aoa = 0.6, 0.0
+    amp = np.deg2rad(0.5)
+    alpha, alpha_dot = amp * np.sin(0.4), amp * 0.2 * np.cos(0.4)
+    X, V, U, M, N = make_case(257, 129, 0.04, mach, aoa, alpha, alpha_dot, 20.0, BATCH)
+    g, p = K.Grid(M, N), K.Params(aoa_deg=aoa)
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(np.swapaxes(a, -1, -2))).cuda()
+    Xd, Vd, Ud = dev(X), dev(V), dev(U)
+    Rd = torch.empty_like(Ud)
+    for _ in range(5):
+        K.compute_odes_batch(p, mach, g, Xd, Vd, Ud, Rd)
+    X, V, U, M, N = make_case(1025, 513, 0.04, 0.5, 0.0, 0.0, 0.0, 30.0, 1)
+    g, p = K.Grid(M, N), K.Params(aoa_deg=0.0)
+    Xd, Vd, Ud = dev(X), dev(V), dev(U[0])
+    rng = np.random.default_rng(1)
+    cshape = (4, M, N)
+    dU = dev(rng.standard_normal(cshape) + 1j * rng.standard_normal(cshape))
+    w = dev(rng.standard_normal(cshape) + 1j * rng.standard_normal(cshape))
+    out = torch.empty_like(dU)
+    R1 = torch.empty(g.cell_shape, dtype=torch.float64, device="cuda")
+    for _ in range(5):
+        K.compute_odes(p, 0.5, g, Xd, Vd, Ud, R1)
+        K.jvp_odes(p, 0.5, g, Xd, Vd, Ud, dU=dU, dR=out)
+        K.vjp_odes(p, 0.5, g, Xd, Vd, Ud, w)
+        K.cs_jvp_odes(p, 0.5, g, Xd, Vd, Ud, dU=dU, dR=out)
+    torch.cuda.synchronize()
+
+
 def cpu_baseline(seconds, X, V, U, mach, aoa):
     """The reference's compiled Fortran compute_odes on one host core (bounded sample)."""
     from oracle.oracle import Oracle, Params
@@ -219,6 +259,10 @@ def main():
 
     from pyeulerale_amd import kernels as K
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.profile:
+        profile_launches(torch, K)
+        progress("profile launches done")
+        return
     mach, aoa = 0.6, 0.0
     omega, t_ = 0.2, 2.0
     amp = np.deg2rad(0.5)

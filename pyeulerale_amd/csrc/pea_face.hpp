@@ -57,34 +57,33 @@ PEA_HD Dual<K> abs_signed(const Dual<K>& q, double s) {
 // p* and c* are the cells' pressures and sound speeds; ll/rr (and their pressures) are only
 // read by a dissipative side.  f_L and f_R are the contributions to the residual sums of L and
 // R, those of face_flux() evaluated from each side.
+// Switch coefficients of one side (f90:207-213): e2 and e4 with D = e2 d2 - e4 d4.
+template <class T>
+PEA_HD void side_coefs(const pea_params& P, const T& lam_l, const T& lam_r, const T& s1,
+                       const T& s2, T& e2, T& e4) {
+    const T lam = maxb(lam_l, lam_r);
+    const T s = maxb(s1, s2);
+    e2 = P.k2 * s * lam;
+    e4 = maxb(lit<T>(0.0), P.k4 * lam - P.c4 * e2);
+}
+
 template <class T, class G>
 PEA_HD void face_flux2(const pea_params& P, const T* LL, const T* L, const T* R, const T* RR,
                        const T& pLL, const T& pL, const T& pR, const T& pRR, const T& cL,
                        const T& cR, const G* n, const G* nR, const G* gv, bool dissL,
                        bool dissR, T* fL, T* fR) {
+    // scalar coefficients first, then one component at a time: only the stencil states, the
+    // coefficients and the current component are live (the tangent types are register-bound)
     const T qL = (n[0] * L[1] + n[1] * L[2]) / L[0];
     const T qR = (n[0] * R[1] + n[1] * R[2]) / R[0];
     const G vn = gv[0] * n[0] + gv[1] * n[1];
-    T cen[4], ale[4];
-    {
-        const T phL[4] = {L[0] * qL, L[1] * qL + n[0] * pL, L[2] * qL + n[1] * pL,
-                          (L[3] + pL) * qL};
-        const T phR[4] = {R[0] * qR, R[1] * qR + n[0] * pR, R[2] * qR + n[1] * pR,
-                          (R[3] + pR) * qR};
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            cen[v] = (phL[v] + phR[v]) / 2.0;
-            ale[v] = (L[v] + R[v]) / 2.0 * vn;
-        }
-    }
-    T DL[4], DR[4];
+    T e2L = lit<T>(0.0), e4L = lit<T>(0.0), e2R = lit<T>(0.0), e4R = lit<T>(0.0);
     if (dissL || dissR) {
         const G sn = ssqrt(n[0] * n[0] + n[1] * n[1]);
         if (dissL) {
             const T lamL = sabs(qL) + cL * sn;
             const T lamR = sabs(qR) + cR * sn;
-            side_dissipation(P, lamL, lamR, sensor(pLL, pL, pR), sensor(pL, pR, pRR), LL, L, R,
-                             RR, DL);
+            side_coefs(P, lamL, lamR, sensor(pLL, pL, pR), sensor(pL, pR, pRR), e2L, e4L);
         }
         if (dissR) {
             // R's view: q' = (nR . j) / rho = -q up to the sign of an exact zero
@@ -92,18 +91,23 @@ PEA_HD void face_flux2(const pea_params& P, const T* LL, const T* L, const T* R,
             const double sR = -sign1(val(nR[0]) * val(R[1]) + val(nR[1]) * val(R[2]));
             const T lamL = abs_signed(qL, sL) + cL * sn;
             const T lamR = abs_signed(qR, sR) + cR * sn;
-            side_dissipation(P, lamR, lamL, sensor(pRR, pR, pL), sensor(pR, pL, pLL), RR, R, L,
-                             LL, DR);
+            side_coefs(P, lamR, lamL, sensor(pRR, pR, pL), sensor(pR, pL, pLL), e2R, e4R);
         }
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        T a = cen[v];
-        if (dissL) a = a - DL[v];
-        fL[v] = a - ale[v];
-        T b = -cen[v];
-        if (dissR) b = b - DR[v];
-        fR[v] = b + ale[v];
+        const T phL = v == 0 ? L[0] * qL
+                             : v == 3 ? (L[3] + pL) * qL : L[v] * qL + n[v - 1] * pL;
+        const T phR = v == 0 ? R[0] * qR
+                             : v == 3 ? (R[3] + pR) * qR : R[v] * qR + n[v - 1] * pR;
+        const T cen = (phL + phR) / 2.0;
+        const T ale = (L[v] + R[v]) / 2.0 * vn;
+        T a = cen;
+        if (dissL) a = a - (e2L * (R[v] - L[v]) - e4L * (RR[v] - 3.0 * R[v] + 3.0 * L[v] - LL[v]));
+        fL[v] = a - ale;
+        T b = -cen;
+        if (dissR) b = b - (e2R * (L[v] - R[v]) - e4R * (LL[v] - 3.0 * L[v] + 3.0 * R[v] - RR[v]));
+        fR[v] = b + ale;
     }
 }
 

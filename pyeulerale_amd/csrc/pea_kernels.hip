@@ -309,13 +309,34 @@ k_jvp(pea_params P, Ld ld, double* __restrict__ dR, double* __restrict__ R) {
 // into LDS; the wall and far-field ghosts sit in the grown rows m = -1 and m = M.  Each cell
 // then sums its four face fluxes in the reference's order (b, f, i, o; f90:395-433).
 // blockIdx.z selects one of several states on the same grid (consecutive 4*C arrays).
+// A value of scalar type T (double, Dual<K>, Cx: plain aggregates of doubles) stored in LDS as
+// structure of arrays: component k of element i at base[k * stride + i].
+template <class T>
+PEA_HD void lds_put(double* base, int stride, int i, const T& x) {
+    constexpr int W = sizeof(T) / sizeof(double);
+    const double* p = reinterpret_cast<const double*>(&x);
+#pragma unroll
+    for (int k = 0; k < W; ++k) base[k * stride + i] = p[k];
+}
+template <class T>
+PEA_HD T lds_get(const double* base, int stride, int i) {
+    constexpr int W = sizeof(T) / sizeof(double);
+    T x;
+    double* p = reinterpret_cast<double*>(&x);
+#pragma unroll
+    for (int k = 0; k < W; ++k) p[k] = base[k * stride + i];
+    return x;
+}
+
 template <class T, class G, int TM, int TN, class Ld, class Store>
 __global__ void __launch_bounds__(TM * TN)
 k_face_fwd(pea_params P, Ld ld, Store store) {
     constexpr int NT = TM * TN, BM = TM + 4, BN = TN + 4, BOX = BM * BN;
     __shared__ T sP[BOX], sC[BOX];
     __shared__ T sG[2][BN][4];
-    __shared__ T sFb[NT][4], sFi[NT][4];
+    // fluxes handed to the neighbouring cell, structure of arrays of the doubles of T
+    constexpr int W = sizeof(T) / sizeof(double);
+    __shared__ double sFb[4 * W * NT], sFi[4 * W * NT];
     const Grid g = ld.g;
     Ld l = ld;
     l.U = ld.U + (long)blockIdx.z * 4 * g.C;
@@ -378,7 +399,7 @@ k_face_fwd(pea_params P, Ld ld, Store store) {
     __syncthreads();
 
     auto bidx = [&](int mm, int nb) { return nb * BM + (mm - m0 + 2); };   // nb: box column
-    T Ff[4], Fo[4], Fb[4], Fi[4];
+    T S[4];   // running sum of this cell's four face fluxes
     const int bn = tn + 2;
     if (live) {
         T uc[4];
@@ -395,14 +416,14 @@ k_face_fwd(pea_params P, Ld ld, Store store) {
             nrm(m + 1, n + 1, m, n + 1, nr2);         // R's bo -> bi
             face_flux2(P, LLs, uc, Rs, RRs, sP[bidx(m, bn - 1)], sP[bidx(m, bn)],
                        sP[bidx(m, bn + 1)], sP[bidx(m, bn + 2)], sC[bidx(m, bn)],
-                       sC[bidx(m, bn + 1)], nr, nr2, gv, true, true, Ff, fR);
+                       sC[bidx(m, bn + 1)], nr, nr2, gv, true, true, S, fR);
             if (tn + 1 < TN && n + 1 < g.N)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) sFb[tid + TM][v] = fR[v];
+                for (int v = 0; v < 4; ++v) lds_put(sFb + v * W * NT, NT, tid + TM, fR[v]);
         }
         // ---- radial face m | m+1 (this cell's o face), or the far-field face
         if (m < g.M - 1) {
-            T Rs[4], LLs[4], RRs[4], fR[4];
+            T Rs[4], LLs[4], RRs[4], fR[4], Fo[4];
             l.state(c + 1, Rs);
             if (m == 0) {
 #pragma unroll
@@ -422,20 +443,24 @@ k_face_fwd(pea_params P, Ld ld, Store store) {
             face_flux2(P, LLs, uc, Rs, RRs, sP[bidx(m - 1, bn)], sP[bidx(m, bn)],
                        sP[bidx(m + 1, bn)], sP[bidx(m + 2, bn)], sC[bidx(m, bn)],
                        sC[bidx(m + 1, bn)], nr, nr2, gv, m != g.M - 2, m + 1 != 1, Fo, fR);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) S[v] = S[v] + Fo[v];
             if (tm + 1 < TM)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) sFi[tid + 1][v] = fR[v];
+                for (int v = 0; v < 4; ++v) lds_put(sFi + v * W * NT, NT, tid + 1, fR[v]);
         } else {
-            T gh[4];
+            T gh[4], Fo[4];
 #pragma unroll
             for (int v = 0; v < 4; ++v) gh[v] = sG[1][bn][v];
             G nr[2], gv[2];
             edge(m + 1, n + 1, m + 1, n, nr, gv);
             face_flux(P, uc, uc, gh, gh, nr, gv, false, Fo);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) S[v] = S[v] + Fo[v];
         }
         // ---- faces shared with the neighbouring tiles
         if (tn == 0) {   // angular face n-1 | n; normal L -> R is minus this cell's b normal
-            T Ls[4], LLs[4], RRs[4], fL[4];
+            T Ls[4], LLs[4], RRs[4], fL[4], Fb[4];
             l.state(wrapn(n - 1, g.N) * g.M + m, Ls);
             l.state(wrapn(n - 2, g.N) * g.M + m, LLs);
             l.state(wrapn(n + 1, g.N) * g.M + m, RRs);
@@ -445,17 +470,21 @@ k_face_fwd(pea_params P, Ld ld, Store store) {
             face_flux2(P, LLs, Ls, uc, RRs, sP[bidx(m, bn - 2)], sP[bidx(m, bn - 1)],
                        sP[bidx(m, bn)], sP[bidx(m, bn + 1)], sC[bidx(m, bn - 1)],
                        sC[bidx(m, bn)], nr, nr2, gv, false, true, fL, Fb);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) S[v] = S[v] + Fb[v];
         }
         if (tm == 0) {
             if (m == 0) {   // wall face: central against the wall ghost
-                T gh[4];
+                T gh[4], Fi[4];
 #pragma unroll
                 for (int v = 0; v < 4; ++v) gh[v] = sG[0][bn][v];
                 G nr[2], gv[2];
                 edge(0, n, 0, n + 1, nr, gv);          // bi -> fi
                 face_flux(P, uc, uc, gh, gh, nr, gv, false, Fi);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) S[v] = S[v] + Fi[v];
             } else {        // radial face m-1 | m; normal L -> R is minus this cell's i normal
-                T Ls[4], LLs[4], RRs[4], fL[4];
+                T Ls[4], LLs[4], RRs[4], fL[4], Fi[4];
                 l.state(c - 1, Ls);
                 if (m - 1 == 0) {
 #pragma unroll
@@ -475,6 +504,8 @@ k_face_fwd(pea_params P, Ld ld, Store store) {
                 face_flux2(P, LLs, Ls, uc, RRs, sP[bidx(m - 2, bn)], sP[bidx(m - 1, bn)],
                            sP[bidx(m, bn)], sP[bidx(m + 1, bn)], sC[bidx(m - 1, bn)],
                            sC[bidx(m, bn)], nr, nr2, gv, false, m != 1, fL, Fi);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) S[v] = S[v] + Fi[v];
             }
         }
     }
@@ -482,10 +513,10 @@ k_face_fwd(pea_params P, Ld ld, Store store) {
     if (live) {
         if (tn > 0)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) Fb[v] = sFb[tid][v];
+            for (int v = 0; v < 4; ++v) S[v] = S[v] + lds_get<T>(sFb + v * W * NT, NT, tid);
         if (tm > 0)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) Fi[v] = sFi[tid][v];
+            for (int v = 0; v < 4; ++v) S[v] = S[v] + lds_get<T>(sFi + v * W * NT, NT, tid);
         G x[4][2];
         vtx(m, n, x[BI]);
         vtx(m, n + 1, x[FI]);
@@ -496,7 +527,7 @@ k_face_fwd(pea_params P, Ld ld, Store store) {
         const G area = sabs(d1x * d2z - d1z * d2x) / 2.0;
         T R[4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) R[v] = -((((Fb[v] + Ff[v]) + Fi[v]) + Fo[v]) / area);
+        for (int v = 0; v < 4; ++v) R[v] = -(S[v] / area);
         store.put(blockIdx.z, n * g.M + m, R);
     }
 }
@@ -1158,9 +1189,10 @@ k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __
     constexpr int L = 2;
     constexpr int BM = TM + 6, BN = TN + 6, BOX = BM * BN, NC = TM * TN;
     __shared__ T sP[BOX], sC[BOX];
-    __shared__ T sFB[BOX][4][L];          // -w / area of each cell of the grown tile
+    // structure-of-arrays: consecutive lanes touch consecutive doubles (no bank conflicts)
+    __shared__ T sFB[4][L][BOX];          // -w / area of each cell of the grown tile
     __shared__ T sG[2][BN][4];            // wall (m = -1) and far-field (m = M) ghosts
-    __shared__ T accU[NC][4][L], accP[NC][L], accC[NC][L];
+    __shared__ T accU[4][L][NC], accP[L][NC], accC[L][NC];
     __shared__ T gU_[2][TN][4][L], gP_[2][TN][L];   // ghost adjoints of the tile's columns
     const Grid g = ld.g;
     const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
@@ -1190,6 +1222,17 @@ k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __
         return d;
     };
     auto in_tile_n = [&](int nn) { return nn >= n0 && nn < n0 + TN; };
+    struct Seed {
+        T w[4][L];
+    };
+    auto seed = [&](int b) {
+        Seed r;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int kk = 0; kk < L; ++kk) r.w[v][kk] = sFB[v][kk][b];
+        return r;
+    };
 
     // 1. per-cell pressures, sound speeds, seeds; ghosts; zero accumulators
     for (int i = tid; i < BOX; i += NT) {
@@ -1211,8 +1254,8 @@ k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __
             const T ia = 1.0 / (sabs(d1x * d2z - d1z * d2x) / 2.0);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                sFB[i][v][0] = -(wseed[2 * ((long)v * g.C + c)] * ia);
-                sFB[i][v][1] = -(wseed[2 * ((long)v * g.C + c) + 1] * ia);
+                sFB[v][0][i] = -(wseed[2 * ((long)v * g.C + c)] * ia);
+                sFB[v][1][i] = -(wseed[2 * ((long)v * g.C + c) + 1] * ia);
             }
         } else if (cm == -1 || cm == g.M) {
             const int mc = cm < 0 ? 0 : g.M - 1;
@@ -1246,9 +1289,9 @@ k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __
 #pragma unroll
         for (int k = 0; k < L; ++k) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) accU[t][v][k] += a.u[j][v][k];
-            accP[t][k] += a.p[j][k];
-            if (j == 1 || j == 2) accC[t][k] += a.c[j - 1][k];
+            for (int v = 0; v < 4; ++v) accU[v][k][t] += a.u[j][v][k];
+            accP[k][t] += a.p[j][k];
+            if (j == 1 || j == 2) accC[k][t] += a.c[j - 1][k];
         }
     };
 
@@ -1282,16 +1325,16 @@ k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __
                     // side with its own geometry
                     face_flux2_rev<T, T, L>(P, LLs, Ls, Rs, RRs, sP[bLL], sP[bL], sP[bR],
                                             sP[bRR], sC[bL], sC[bR], nL, nRn, gv, true, false,
-                                            sFB[bL], zero, a);
+                                            seed(bL).w, zero, a);
                     T nL0[2], gv0[2];
                     edge(m, 0, m + 1, 0, nL0, gv0);
                     face_flux2_rev<T, T, L>(P, LLs, Ls, Rs, RRs, sP[bLL], sP[bL], sP[bR],
                                             sP[bRR], sC[bL], sC[bR], nL0, nRn, gv0, false, true,
-                                            zero, sFB[bR], a);
+                                            zero, seed(bR).w, a);
                 } else {
                     face_flux2_rev<T, T, L>(P, LLs, Ls, Rs, RRs, sP[bLL], sP[bL], sP[bR],
                                             sP[bRR], sC[bL], sC[bR], nL, nRn, gv, true, true,
-                                            sFB[bL], sFB[bR], a);
+                                            seed(bL).w, seed(bR).w, a);
                 }
             }
 #pragma unroll
@@ -1339,7 +1382,8 @@ k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __
                                         sP[bidx(Lm, bn)], sP[bidx(Lm + 1, bn)],
                                         sP[bidx(Lm + 2, bn)], sC[bidx(Lm, bn)],
                                         sC[bidx(Lm + 1, bn)], nL, nRn, gv, Lm != g.M - 2,
-                                        Lm + 1 != 1, sFB[bidx(Lm, bn)], sFB[bidx(Lm + 1, bn)], a);
+                                        Lm + 1 != 1, seed(bidx(Lm, bn)).w,
+                                        seed(bidx(Lm + 1, bn)).w, a);
             }
 #pragma unroll
             for (int jj = 0; jj < 4; ++jj) {
@@ -1383,9 +1427,9 @@ k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __
 #pragma unroll
         for (int k = 0; k < L; ++k) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) ub[v][k] = accU[t][v][k];
-            pb[k] = accP[t][k];
-            cb[k] = accC[t][k];
+            for (int v = 0; v < 4; ++v) ub[v][k] = accU[v][k][t];
+            pb[k] = accP[k][t];
+            cb[k] = accC[k][t];
         }
         if (m == 0 || m == g.M - 1) {
             const int side = m == 0 ? 0 : 1;
@@ -1401,7 +1445,7 @@ k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __
             const T zero[4][L] = {};
             face_flux2_rev<T, T, L>(P, u, u, gh, gh, sP[b], sP[b], sP[bidx(side ? g.M : -1, bn)],
                                     sP[bidx(side ? g.M : -1, bn)], sC[b], sC[b], nr, nr, gv,
-                                    false, false, sFB[b], zero, a);
+                                    false, false, seed(b).w, zero, a);
             T ghb[4][L], ghp[L];
 #pragma unroll
             for (int k = 0; k < L; ++k) {
