@@ -90,6 +90,24 @@ def graph_time_ms(torch, fn, reps=20):
     return a.elapsed_time(b) / (3 * reps)
 
 
+PMC_SUMMARY = ROOT / "profiles" / "pmc_face_kernels_1025x513.md"
+
+
+def pmc_figures() -> dict:
+    """HBM traffic and FP64 flops per launch of the batched residual kernel, as measured by the
+    rocprofv3 counter passes summarised in profiles/ (FETCH_SIZE and WRITE_SIZE calibrated
+    against a 512 MiB copy; SQ_INSTS_VALU_*_F64).  Empty when the summary is absent."""
+    out = {}
+    try:
+        for line in PMC_SUMMARY.read_text().splitlines():
+            for key in ("traffic_bytes_per_launch", "fp64_flop_per_launch"):
+                if line.startswith(key + " k_face_fwd batched residual"):
+                    out[key] = float(line.rsplit(":", 1)[1])
+    except OSError:
+        pass
+    return out
+
+
 def progress(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -303,6 +321,7 @@ def main():
     if not args.no_freq:
         extra["frequency_response_129x129"] = freq_response(torch, not args.no_superlu)
         progress("frequency response done")
+    pmc = pmc_figures()
     line = {
         "metric": "residual evals/s + hand-written VJP vs complex-step JVP speedup (FP64, 1 GPU)",
         "value": value, "unit": "residual evals/s", "n_gpus": 1, "steps": args.steps,
@@ -312,9 +331,15 @@ def main():
                                "Ma 0.6, batch of 64 states per launch",
                    "batch": BATCH, "grid": "257x129"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_odes", "kernel_ms": kern_ms,
-                     "algorithmic_bytes_per_launch": bytes_launch},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": pmc.get("traffic_bytes_per_launch"),
+                     "kernel": "k_face_fwd<double, double, 32, 8, LdReal, StoreRealB>",
+                     "kernel_ms": kern_ms, "algorithmic_bytes_per_launch": bytes_launch,
+                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/"
+                                       "pmc_face_kernels_1025x513.md",
+                     "fp64_tflops": (pmc["fp64_flop_per_launch"] / (kern_ms * 1e-3) / 1e12
+                                     if "fp64_flop_per_launch" in pmc else None),
+                     "fp64_peak_tflops": FP64_PEAK_TFLOPS},
         "speedup_vjp_vs_cs_jvp": extra["products_257x129"]["speedup_vjp_vs_cs_dir"],
         "extra": extra,
     }
