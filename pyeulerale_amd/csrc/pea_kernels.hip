@@ -332,10 +332,13 @@ template <class T, class G, int TM, int TN, class Ld, class Store>
 __global__ void __launch_bounds__(TM * TN)
 k_face_fwd(pea_params P, Ld ld, Store store) {
     constexpr int NT = TM * TN, BM = TM + 4, BN = TN + 4, BOX = BM * BN;
-    __shared__ T sP[BOX], sC[BOX];
-    __shared__ T sG[2][BN][4];
-    // fluxes handed to the neighbouring cell, structure of arrays of the doubles of T
+    // LDS holds, as structure of arrays of the doubles of T: the states of the tile grown by two
+    // cells (with the wall and far-field ghosts in the grown rows m = -1 and m = M), and the
+    // fluxes handed to the neighbouring cell.  Every stencil read of the face evaluations is an
+    // LDS read; the global loads happen once, coalesced, in the staging loop.
     constexpr int W = sizeof(T) / sizeof(double);
+    __shared__ T sP[BOX], sC[BOX];
+    __shared__ double sU[4 * W * BOX];
     __shared__ double sFb[4 * W * NT], sFi[4 * W * NT];
     const Grid g = ld.g;
     Ld l = ld;
@@ -377,6 +380,8 @@ k_face_fwd(pea_params P, Ld ld, Store store) {
             T u[4];
             l.state(cn * g.M + cm, u);
             cell_pc(u, sP[i], sC[i]);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) lds_put(sU + v * W * BOX, BOX, i, u[v]);
         } else if (cm == -1 || cm == g.M) {
             const int mc = cm < 0 ? 0 : g.M - 1;
             T u[4], gh[4];
@@ -392,25 +397,30 @@ k_face_fwd(pea_params P, Ld ld, Store store) {
                 for (int v = 0; v < 4; ++v) gh[v] = 2.0 * inf[v] - u[v];
             }
 #pragma unroll
-            for (int v = 0; v < 4; ++v) sG[cm < 0 ? 0 : 1][bn][v] = gh[v];
+            for (int v = 0; v < 4; ++v) lds_put(sU + v * W * BOX, BOX, i, gh[v]);
             sP[i] = pressure(gh);
         }
     }
     __syncthreads();
 
     auto bidx = [&](int mm, int nb) { return nb * BM + (mm - m0 + 2); };   // nb: box column
+    // state of cell (mm, box column nb); mm = -1 and mm = M are the ghosts
+    auto box = [&](int mm, int nb, T* u) {
+        const int b = bidx(mm, nb);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) u[v] = lds_get<T>(sU + v * W * BOX, BOX, b);
+    };
     T S[4];   // running sum of this cell's four face fluxes
     const int bn = tn + 2;
     if (live) {
         T uc[4];
-        const int c = n * g.M + m;
-        l.state(c, uc);
+        box(m, bn, uc);
         // ---- angular face n | n+1 (this cell's f face)
         {
             T Rs[4], LLs[4], RRs[4], fR[4];
-            l.state(wrapn(n + 1, g.N) * g.M + m, Rs);
-            l.state(wrapn(n - 1, g.N) * g.M + m, LLs);
-            l.state(wrapn(n + 2, g.N) * g.M + m, RRs);
+            box(m, bn + 1, Rs);
+            box(m, bn - 1, LLs);
+            box(m, bn + 2, RRs);
             G nr[2], nr2[2], gv[2];
             edge(m, n + 1, m + 1, n + 1, nr, gv);     // fi -> fo
             nrm(m + 1, n + 1, m, n + 1, nr2);         // R's bo -> bi
@@ -424,19 +434,9 @@ k_face_fwd(pea_params P, Ld ld, Store store) {
         // ---- radial face m | m+1 (this cell's o face), or the far-field face
         if (m < g.M - 1) {
             T Rs[4], LLs[4], RRs[4], fR[4], Fo[4];
-            l.state(c + 1, Rs);
-            if (m == 0) {
-#pragma unroll
-                for (int v = 0; v < 4; ++v) LLs[v] = sG[0][bn][v];
-            } else {
-                l.state(c - 1, LLs);
-            }
-            if (m + 1 == g.M - 1) {
-#pragma unroll
-                for (int v = 0; v < 4; ++v) RRs[v] = sG[1][bn][v];
-            } else {
-                l.state(c + 2, RRs);
-            }
+            box(m + 1, bn, Rs);
+            box(m - 1, bn, LLs);    // the wall ghost for m = 0
+            box(m + 2, bn, RRs);    // the far-field ghost for m + 1 = M - 1
             G nr[2], nr2[2], gv[2];
             edge(m + 1, n + 1, m + 1, n, nr, gv);     // fo -> bo
             nrm(m + 1, n, m + 1, n + 1, nr2);         // R's bi -> fi
@@ -450,8 +450,7 @@ k_face_fwd(pea_params P, Ld ld, Store store) {
                 for (int v = 0; v < 4; ++v) lds_put(sFi + v * W * NT, NT, tid + 1, fR[v]);
         } else {
             T gh[4], Fo[4];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) gh[v] = sG[1][bn][v];
+            box(g.M, bn, gh);
             G nr[2], gv[2];
             edge(m + 1, n + 1, m + 1, n, nr, gv);
             face_flux(P, uc, uc, gh, gh, nr, gv, false, Fo);
@@ -461,9 +460,9 @@ k_face_fwd(pea_params P, Ld ld, Store store) {
         // ---- faces shared with the neighbouring tiles
         if (tn == 0) {   // angular face n-1 | n; normal L -> R is minus this cell's b normal
             T Ls[4], LLs[4], RRs[4], fL[4], Fb[4];
-            l.state(wrapn(n - 1, g.N) * g.M + m, Ls);
-            l.state(wrapn(n - 2, g.N) * g.M + m, LLs);
-            l.state(wrapn(n + 1, g.N) * g.M + m, RRs);
+            box(m, bn - 1, Ls);
+            box(m, bn - 2, LLs);
+            box(m, bn + 1, RRs);
             G nr[2], nr2[2], gv[2];
             edge(m, n, m + 1, n, nr, gv);             // bi -> bo
             nrm(m + 1, n, m, n, nr2);                 // this cell's bo -> bi
@@ -476,8 +475,7 @@ k_face_fwd(pea_params P, Ld ld, Store store) {
         if (tm == 0) {
             if (m == 0) {   // wall face: central against the wall ghost
                 T gh[4], Fi[4];
-#pragma unroll
-                for (int v = 0; v < 4; ++v) gh[v] = sG[0][bn][v];
+                box(-1, bn, gh);
                 G nr[2], gv[2];
                 edge(0, n, 0, n + 1, nr, gv);          // bi -> fi
                 face_flux(P, uc, uc, gh, gh, nr, gv, false, Fi);
@@ -485,19 +483,9 @@ k_face_fwd(pea_params P, Ld ld, Store store) {
                 for (int v = 0; v < 4; ++v) S[v] = S[v] + Fi[v];
             } else {        // radial face m-1 | m; normal L -> R is minus this cell's i normal
                 T Ls[4], LLs[4], RRs[4], fL[4], Fi[4];
-                l.state(c - 1, Ls);
-                if (m - 1 == 0) {
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) LLs[v] = sG[0][bn][v];
-                } else {
-                    l.state(c - 2, LLs);
-                }
-                if (m == g.M - 1) {
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) RRs[v] = sG[1][bn][v];
-                } else {
-                    l.state(c + 1, RRs);
-                }
+                box(m - 1, bn, Ls);
+                box(m - 2, bn, LLs);    // the wall ghost for m = 1
+                box(m + 1, bn, RRs);    // the far-field ghost for m = M - 1
                 G nr[2], nr2[2], gv[2];
                 edge(m, n + 1, m, n, nr, gv);          // fo -> bo of L
                 nrm(m, n, m, n + 1, nr2);              // this cell's bi -> fi
