@@ -1179,7 +1179,9 @@ k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __
     __shared__ T sP[BOX], sC[BOX];
     // structure-of-arrays: consecutive lanes touch consecutive doubles (no bank conflicts)
     __shared__ T sFB[4][L][BOX];          // -w / area of each cell of the grown tile
-    __shared__ T sG[2][BN][4];            // wall (m = -1) and far-field (m = M) ghosts
+    // states of the grown tile, wall (m = -1) and far-field (m = M) ghosts in the grown rows,
+    // staged once so that every stencil read below is an LDS read
+    __shared__ T sU[4][BOX];
     __shared__ T accU[4][L][NC], accP[L][NC], accC[L][NC];
     __shared__ T gU_[2][TN][4][L], gP_[2][TN][L];   // ghost adjoints of the tile's columns
     const Grid g = ld.g;
@@ -1210,6 +1212,10 @@ k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __
         return d;
     };
     auto in_tile_n = [&](int nn) { return nn >= n0 && nn < n0 + TN; };
+    auto box = [&](int b, T* u) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) u[v] = sU[v][b];
+    };
     struct Seed {
         T w[4][L];
     };
@@ -1232,6 +1238,8 @@ k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __
             const int c = cn * g.M + cm;
             ld.state(c, u);
             cell_pc(u, sP[i], sC[i]);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sU[v][i] = u[v];
             T x[4][2];
             vtx(cm, cn, x[BI]);
             vtx(cm, cn + 1, x[FI]);
@@ -1260,7 +1268,7 @@ k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __
                 for (int v = 0; v < 4; ++v) gh[v] = 2.0 * inf[v] - u[v];
             }
 #pragma unroll
-            for (int v = 0; v < 4; ++v) sG[cm < 0 ? 0 : 1][bn][v] = gh[v];
+            for (int v = 0; v < 4; ++v) sU[v][i] = gh[v];
             sP[i] = pressure(gh);
         }
     }
@@ -1297,13 +1305,13 @@ k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __
                 face_adj_zero(a);
                 const int Rc = wrapn(Lc + 1, g.N), LLc = wrapn(Lc - 1, g.N),
                           RRc = wrapn(Lc + 2, g.N);
-                T Ls[4], Rs[4], LLs[4], RRs[4];
-                ld.state(Lc * g.M + m, Ls);
-                ld.state(Rc * g.M + m, Rs);
-                ld.state(LLc * g.M + m, LLs);
-                ld.state(RRc * g.M + m, RRs);
                 const int bL = bidx(m, bcol(Lc)), bR = bidx(m, bcol(Rc));
                 const int bLL = bidx(m, bcol(LLc)), bRR = bidx(m, bcol(RRc));
+                T Ls[4], Rs[4], LLs[4], RRs[4];
+                box(bL, Ls);
+                box(bR, Rs);
+                box(bLL, LLs);
+                box(bRR, RRs);
                 T nL[2], nRn[2], gv[2];
                 edge(m, Lc + 1, m + 1, Lc + 1, nL, gv);     // L's fi -> fo
                 nrm(m + 1, Rc, m, Rc, nRn);                 // R's bo -> bi
@@ -1348,21 +1356,10 @@ k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __
                 face_adj_zero(a);
                 const int bn = tn + 3;
                 T Ls[4], Rs[4], LLs[4], RRs[4];
-                const int c = n * g.M + Lm;
-                ld.state(c, Ls);
-                ld.state(c + 1, Rs);
-                if (Lm == 0) {
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) LLs[v] = sG[0][bn][v];
-                } else {
-                    ld.state(c - 1, LLs);
-                }
-                if (Lm + 1 == g.M - 1) {
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) RRs[v] = sG[1][bn][v];
-                } else {
-                    ld.state(c + 2, RRs);
-                }
+                box(bidx(Lm, bn), Ls);
+                box(bidx(Lm + 1, bn), Rs);
+                box(bidx(Lm - 1, bn), LLs);    // the wall ghost for Lm = 0
+                box(bidx(Lm + 2, bn), RRs);    // the far-field ghost for Lm + 1 = M - 1
                 T nL[2], nRn[2], gv[2];
                 edge(Lm + 1, n + 1, Lm + 1, n, nL, gv);      // L's fo -> bo
                 nrm(Lm + 1, n, Lm + 1, n + 1, nRn);          // R's bi -> fi
@@ -1410,7 +1407,7 @@ k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __
         if (m >= g.M || n >= g.N) continue;
         const int c = n * g.M + m, bn = tn + 3, b = bidx(m, bn);
         T u[4];
-        ld.state(c, u);
+        box(b, u);
         T ub[4][L], pb[L], cb[L];
 #pragma unroll
         for (int k = 0; k < L; ++k) {
@@ -1422,8 +1419,7 @@ k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __
         if (m == 0 || m == g.M - 1) {
             const int side = m == 0 ? 0 : 1;
             T gh[4];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) gh[v] = sG[side][bn][v];
+            box(bidx(side ? g.M : -1, bn), gh);
             T nr[2], gv[2];
             if (side == 0) edge(0, n, 0, n + 1, nr, gv);           // wall face bi -> fi
             else edge(m + 1, n + 1, m + 1, n, nr, gv);             // far field fo -> bo
