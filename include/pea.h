@@ -33,6 +33,17 @@ typedef struct pea_params {
     double c4;        /* JST c_4     (f90:66), default 2     */
 } pea_params;
 
+/* One grid of a multi-grid launch: dimensions, flow parameters and device arrays in the
+ * layouts above (R is written; dU/dR are the tangent direction and output of the multi-grid
+ * JVP, complex interleaved, and may be NULL for the residual). */
+typedef struct pea_grid {
+    int M, N;
+    double mach;
+    pea_params prm;
+    const double *X, *V, *U, *dU;
+    double *R, *dR;
+} pea_grid;
+
 enum pea_status {
     PEA_OK = 0,
     PEA_ERR_DIMS = -1,    /* M < 4, N < 5 or sizes overflow */
@@ -66,6 +77,16 @@ int pea_compute_odes(const pea_params *prm, double mach, int M, int N, const dou
 int pea_compute_odes_batch(const pea_params *prm, double mach, int M, int N, int nstates,
                            const double *X, const double *V, const double *U, double *R,
                            void *stream);
+
+/* Many grids of uneven size in one persistent launch (the reference evaluates one grid per
+ * compute_odes call, f90:496-537).  `grids` is a host array of G descriptors; `workspace` is
+ * device memory of pea_multi_workspace_bytes(G) bytes, overwritten (stream-ordered) with the
+ * launch's descriptor table.  Results are bitwise those of G single-grid calls.
+ * pea_jvp_odes_multigrid: the states-direction JVP (pea_jvp_odes with dU only) of every grid,
+ * writing dR (and R when non-NULL). */
+long long pea_multi_workspace_bytes(int G);
+int pea_compute_odes_multi(int G, const pea_grid *grids, void *workspace, void *stream);
+int pea_jvp_odes_multigrid(int G, const pea_grid *grids, void *workspace, void *stream);
 
 /* Surface forces F_n = p(U_{0,n}) normal(X_{0,n} -> X_{0,n+1}).  Replaces compute_forces
  * (f90:1156-1174).  If coef != NULL (device, 3 doubles) it also receives the reductions

@@ -24,6 +24,12 @@ class PeaParams(_c.Structure):
     _fields_ = [("aoa_deg", _D), ("k2", _D), ("k4", _D), ("c4", _D)]
 
 
+class PeaGrid(_c.Structure):
+    """Mirror of ``pea_grid`` (one grid of a multi-grid launch)."""
+    _fields_ = [("M", _c.c_int), ("N", _c.c_int), ("mach", _D), ("prm", PeaParams),
+                ("X", _P), ("V", _P), ("U", _P), ("dU", _P), ("R", _P), ("dR", _P)]
+
+
 # name -> argument types (restype int unless listed in _RESTYPES)
 _SIGS = {
     "pea_last_error": [],
@@ -50,6 +56,9 @@ _SIGS = {
     "pea_apply_odes_wrt_vertices_fwd": [_I, _I, _P, _P, _P, _P],
     "pea_apply_odes_wrt_vertices_rev": [_I, _I, _P, _P, _P, _P],
     "pea_export_bsr": [_I, _I, _P, _D, _D, _I, _P, _P, _P, _P],
+    "pea_multi_workspace_bytes": [_I],
+    "pea_compute_odes_multi": [_I, _P, _P, _P],
+    "pea_jvp_odes_multigrid": [_I, _P, _P, _P],
     "pea_resolvent_last_error": [],
     "pea_block_multi": [_I, _I, _I, _P, _P, _I, _P, _P, _P],
     "pea_line_factor_bytes": [_I, _I, _I],
@@ -63,6 +72,7 @@ _SIGS = {
     "pea_dense_lu_solve": [_I, _I, _I, _P, _P, _P, _P],
 }
 _RESTYPES = {"pea_last_error": _c.c_char_p, "pea_vjp_workspace_bytes": _c.c_longlong,
+             "pea_multi_workspace_bytes": _c.c_longlong,
              "pea_resolvent_last_error": _c.c_char_p, "pea_line_factor_bytes": _c.c_longlong}
 
 _RESOLVENT = {"pea_block_multi", "pea_line_factor", "pea_line_solve", "pea_jacobi_factor",

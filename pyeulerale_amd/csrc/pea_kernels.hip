@@ -13,6 +13,7 @@
 #include <cstring>
 #include <mutex>
 #include <type_traits>
+#include <vector>
 
 #include "pea_adjoint.hpp"
 #include "pea_face.hpp"
@@ -328,9 +329,11 @@ PEA_HD T lds_get(const double* base, int stride, int i) {
     return x;
 }
 
+// One tile of TM x TN cells at tile coordinates (bx, by) of state bz; the body of the
+// single-grid and of the multi-grid kernels.
 template <class T, class G, int TM, int TN, class Ld, class Store>
-__global__ void __launch_bounds__(TM * TN)
-k_face_fwd(pea_params P, Ld ld, Store store) {
+__device__ __forceinline__ void face_fwd_tile(const pea_params& P, const Ld& ld,
+                                              const Store& store, int bx, int by, int bz) {
     constexpr int NT = TM * TN, BM = TM + 4, BN = TN + 4, BOX = BM * BN;
     // LDS holds, as structure of arrays of the doubles of T: the states of the tile grown by two
     // cells (with the wall and far-field ghosts in the grown rows m = -1 and m = M), and the
@@ -342,8 +345,8 @@ k_face_fwd(pea_params P, Ld ld, Store store) {
     __shared__ double sFb[4 * W * NT], sFi[4 * W * NT];
     const Grid g = ld.g;
     Ld l = ld;
-    l.U = ld.U + (long)blockIdx.z * 4 * g.C;
-    const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
+    l.U = ld.U + (long)bz * 4 * g.C;
+    const int m0 = bx * TM, n0 = by * TN;
     const int tid = threadIdx.x;
     const int tm = tid % TM, tn = tid / TM;
     const int m = m0 + tm, n = n0 + tn;
@@ -516,8 +519,14 @@ k_face_fwd(pea_params P, Ld ld, Store store) {
         T R[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) R[v] = -(S[v] / area);
-        store.put(blockIdx.z, n * g.M + m, R);
+        store.put(bz, n * g.M + m, R);
     }
+}
+
+template <class T, class G, int TM, int TN, class Ld, class Store>
+__global__ void __launch_bounds__(TM * TN)
+k_face_fwd(pea_params P, Ld ld, Store store) {
+    face_fwd_tile<T, G, TM, TN>(P, ld, store, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 struct StoreRealB {
@@ -1506,6 +1515,47 @@ int cell_block(long C) { return C >= 256L * 1024 ? 256 : 64; }
 // tiles of the face-based forward kernels (m fastest within a tile)
 constexpr int kFwdTM = 32, kFwdTN = 8;
 constexpr int kJvpTM = 32, kJvpTN = 4;
+
+// ------------------------------------------------------------------ many grids, one launch
+// Descriptor of one grid of a multi-grid launch as the kernel reads it: the public pea_grid
+// plus the grid's first tile in the launch-wide tile numbering.
+struct GridDesc {
+    pea_grid d;
+    int tiles_m, tile0;
+};
+
+constexpr int kXcds = 8;   // MI355X: workgroup i runs on XCD i mod 8, each XCD with its own L2
+
+// Persistent launch over the tiles of G grids.  The tile range is cut into kXcds contiguous
+// segments; the workgroups of one XCD walk one segment, so neighbouring tiles of a grid (which
+// share their halos) are read through the same L2.  gridDim.x is a multiple of kXcds.
+template <bool TANGENT>
+__global__ void __launch_bounds__(TANGENT ? kJvpTM * kJvpTN : kFwdTM * kFwdTN)
+k_face_fwd_multi(const GridDesc* __restrict__ desc, int G, int total) {
+    const int xcd = blockIdx.x % kXcds, lane = blockIdx.x / kXcds, lanes = gridDim.x / kXcds;
+    const int seg = (total + kXcds - 1) / kXcds;
+    const int t_end = min(total, (xcd + 1) * seg);
+    for (int t = xcd * seg + lane; t < t_end; t += lanes) {
+        int lo = 0, hi = G - 1;   // last grid whose first tile is <= t
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (desc[mid].tile0 <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const GridDesc e = desc[lo];
+        const Grid g{e.d.M, e.d.N, e.d.M * e.d.N, (e.d.M + 1) * (e.d.N + 1)};
+        const int tl = t - e.tile0;
+        if (TANGENT)
+            face_fwd_tile<Dual<2>, double, kJvpTM, kJvpTN>(
+                e.d.prm, LdDualU<2>{e.d.X, e.d.V, e.d.U, e.d.dU, e.d.mach, g},
+                StoreDualB<2>{e.d.dR, e.d.R, g.C}, tl % e.tiles_m, tl / e.tiles_m, 0);
+        else
+            face_fwd_tile<double, double, kFwdTM, kFwdTN>(
+                e.d.prm, LdReal{e.d.X, e.d.V, e.d.U, e.d.mach, g}, StoreRealB{e.d.R, g.C},
+                tl % e.tiles_m, tl / e.tiles_m, 0);
+        __syncthreads();   // the next tile restages the LDS
+    }
+}
 dim3 fwd_grid(const Grid& g, int nstates, int tm = kFwdTM, int tn = kFwdTN) {
     return dim3((g.M + tm - 1) / tm, (g.N + tn - 1) / tn, nstates);
 }
@@ -1698,6 +1748,60 @@ int pea_compute_odes_batch(const pea_params* prm, double mach, int M, int N, int
         *prm, LdReal{X, V, U, mach, g}, StoreRealB{R, g.C});
     PEA_LAUNCH_CHECK("k_face_fwd");
     return PEA_OK;
+}
+
+long long pea_multi_workspace_bytes(int G) { return G < 1 ? 0 : (long long)G * sizeof(GridDesc); }
+
+static int multi_launch(int G, const pea_grid* grids, void* workspace, bool tangent,
+                        void* stream) {
+    if (G < 1) return fail(PEA_ERR_DIMS, "no grids");
+    if (!grids || !workspace) return fail(PEA_ERR_NULL, "null argument");
+    const int tm = tangent ? kJvpTM : kFwdTM, tn = tangent ? kJvpTN : kFwdTN;
+    std::vector<GridDesc> d(G);
+    long long total = 0;
+    for (int i = 0; i < G; ++i) {
+        const pea_grid& q = grids[i];
+        if (int rc = check_dims(q.M, q.N)) return rc;
+        if (!q.X || !q.V || !q.U || (tangent ? (!q.dU || !q.dR) : !q.R))
+            return fail(PEA_ERR_NULL, "null array in a grid descriptor");
+        d[i].d = q;
+        d[i].tiles_m = (q.M + tm - 1) / tm;
+        d[i].tile0 = (int)total;
+        total += (long long)d[i].tiles_m * ((q.N + tn - 1) / tn);
+        if (total > 0x7fffffff) return fail(PEA_ERR_DIMS, "too many tiles");
+    }
+    hipStream_t st = as_stream(stream);
+    if (hipMemcpyAsync(workspace, d.data(), G * sizeof(GridDesc), hipMemcpyHostToDevice, st) !=
+        hipSuccess)
+        return fail(PEA_ERR_HIP, "descriptor copy");
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return fail(PEA_ERR_HIP, "device query");
+    hipError_t oc = tangent
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_face_fwd_multi<true>,
+                                                       tm * tn, 0)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_face_fwd_multi<false>,
+                                                       tm * tn, 0);
+    if (oc != hipSuccess || per_cu < 1) per_cu = 1;
+    long long nb = std::min<long long>(total, (long long)cus * per_cu);
+    nb = std::max<long long>(kXcds, (nb + kXcds - 1) / kXcds * kXcds);
+    if (tangent)
+        k_face_fwd_multi<true><<<(unsigned)nb, tm * tn, 0, st>>>(
+            static_cast<const GridDesc*>(workspace), G, (int)total);
+    else
+        k_face_fwd_multi<false><<<(unsigned)nb, tm * tn, 0, st>>>(
+            static_cast<const GridDesc*>(workspace), G, (int)total);
+    PEA_LAUNCH_CHECK("k_face_fwd_multi");
+    return PEA_OK;
+}
+
+int pea_compute_odes_multi(int G, const pea_grid* grids, void* workspace, void* stream) {
+    return multi_launch(G, grids, workspace, false, stream);
+}
+
+int pea_jvp_odes_multigrid(int G, const pea_grid* grids, void* workspace, void* stream) {
+    return multi_launch(G, grids, workspace, true, stream);
 }
 
 int pea_compute_forces(int M, int N, const double* X, const double* U, double* F, double xc,

@@ -135,6 +135,49 @@ def compute_odes_batch(p: Params, mach: float, g: Grid, X, V, U, R=None) -> torc
     return R
 
 
+def _multi(items, tangent: bool):
+    grids = (_lib.PeaGrid * len(items))()
+    outs = []
+    for i, it in enumerate(items):
+        p, mach, X, V, U = it[:5]
+        g = Grid(U.shape[2], U.shape[1])
+        g.check_primal(X, V, U)
+        R = torch.empty(g.cell_shape, dtype=F64, device=U.device)
+        dU = dR = None
+        if tangent:
+            dU = _need(it[5], g.cell_shape, C128, "d_states")
+            dR = torch.empty(g.cell_shape, dtype=C128, device=U.device)
+        grids[i] = _lib.PeaGrid(g.M, g.N, float(mach), p.c(), X.data_ptr(), V.data_ptr(),
+                                U.data_ptr(), dU.data_ptr() if tangent else None, R.data_ptr(),
+                                dR.data_ptr() if tangent else None)
+        outs.append((R, dR) if tangent else R)
+    nbytes = int(_lib.load().pea_multi_workspace_bytes(len(items)))
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=items[0][4].device)
+    # the descriptor table is copied into ws on the stream before the kernel reads it;
+    # keeping ws alive until the launch has run is the caller's stream order
+    _lib.call("pea_jvp_odes_multigrid" if tangent else "pea_compute_odes_multi", len(items),
+              grids, _ptr(ws), _stream())
+    _MULTI_KEEP.append(ws)
+    if len(_MULTI_KEEP) > 8:
+        del _MULTI_KEEP[0]
+    return outs
+
+
+_MULTI_KEEP: list = []
+
+
+def compute_odes_multi(items):
+    """Residuals of many grids of uneven size in one persistent launch.  ``items`` is a list of
+    (params, mach, X, V, U) in storage layout; returns the list of R."""
+    return _multi(items, False)
+
+
+def jvp_odes_multigrid(items):
+    """States-direction tangents of many grids in one launch: items (params, mach, X, V, U, dU);
+    returns the list of (R, dR)."""
+    return _multi(items, True)
+
+
 def compute_forces(g: Grid, X, U, F=None, center=(0.0, 0.0), coef=None):
     _need(X, g.vert_shape, F64, "vertices")
     _need(U, g.cell_shape, F64, "states")
