@@ -42,6 +42,7 @@ def parse():
     ap.add_argument("--no-large", action="store_true", help="skip the 1025x513 products")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-freq", action="store_true", help="skip the frequency-response solve")
+    ap.add_argument("--no-multigrid", action="store_true", help="skip the 64-grid launch")
     ap.add_argument("--no-superlu", action="store_true",
                     help="skip timing the reference's host SuperLU solve")
     ap.add_argument("--profile", action="store_true",
@@ -248,6 +249,40 @@ def profile_launches(torch, K):
     torch.cuda.synchronize()
 
 
+def multigrid(torch, K, G: int = 64, seed: int = 0, reps: int = 10):
+    """64 O-grids of uneven size in one launch against one launch per grid (both replayed from
+    a graph, so no host gaps): NACA 00tt with tt in 04..18, A in 65..513 (odd) angular and R in
+    33..257 radial vertices, Ma in U[0.3, 0.6], alpha in U[-2, 3] deg, rigid pitch of amplitude
+    <= 1 deg, U = U_inf (1 + 1e-3 N(0,1))."""
+    rng = np.random.default_rng(seed)
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(np.swapaxes(a, -1, -2))).cuda()
+    items, cells = [], 0
+    for _ in range(G):
+        tt = int(rng.integers(4, 19))
+        A = 2 * int(rng.integers(32, 257)) + 1
+        Rv = int(rng.integers(33, 258))
+        mach, aoa = float(rng.uniform(0.3, 0.6)), float(rng.uniform(-2.0, 3.0))
+        amp = np.deg2rad(float(rng.uniform(0.0, 1.0)))
+        X, V, U, M, N = make_case(A, Rv, tt / 100.0, mach, aoa, amp * np.sin(0.4),
+                                  amp * 0.2 * np.cos(0.4), 20.0, 1, seed=int(rng.integers(1 << 30)))
+        items.append((K.Params(aoa_deg=aoa), mach, dev(X), dev(V), dev(U[0])))
+        cells += M * N
+    mg = K.MultiGrid(items)
+    singles = [(p, mach, K.Grid(U.shape[2], U.shape[1]), X, V, U, torch.empty_like(U))
+               for p, mach, X, V, U in items]
+
+    def one_by_one():
+        for p, mach, g, X, V, U, R in singles:
+            K.compute_odes(p, mach, g, X, V, U, R)
+    t_multi = graph_time_ms(torch, mg.run, reps)
+    t_single = graph_time_ms(torch, one_by_one, reps)
+    same = all(torch.equal(R, s[6]) for R, s in zip(mg.outputs, singles))
+    return {"grids": G, "cells": cells, "tiles": mg.total_tiles,
+            "ms_multi_launch": t_multi, "ms_one_launch_per_grid": t_single,
+            "speedup": t_single / t_multi, "cells_per_s_multi": cells / (t_multi * 1e-3),
+            "bitwise_equal": bool(same)}
+
+
 def cpu_baseline(seconds, X, V, U, mach, aoa):
     """The reference's compiled Fortran compute_odes on one host core (bounded sample)."""
     from oracle.oracle import Oracle, Params
@@ -318,6 +353,9 @@ def main():
         extra["products_1025x513"] = products(torch, K, 1025, 513, 0.04, 0.5, 0.0, 0.0, 0.0,
                                               30.0, 10)
         progress("products 1025x513 done")
+    if not args.no_multigrid:
+        extra["multigrid_64_grids"] = multigrid(torch, K)
+        progress("multi-grid done")
     if not args.no_freq:
         extra["frequency_response_129x129"] = freq_response(torch, not args.no_superlu)
         progress("frequency response done")

@@ -79,12 +79,18 @@ int pea_compute_odes_batch(const pea_params *prm, double mach, int M, int N, int
                            void *stream);
 
 /* Many grids of uneven size in one persistent launch (the reference evaluates one grid per
- * compute_odes call, f90:496-537).  `grids` is a host array of G descriptors; `workspace` is
- * device memory of pea_multi_workspace_bytes(G) bytes, overwritten (stream-ordered) with the
- * launch's descriptor table.  Results are bitwise those of G single-grid calls.
- * pea_jvp_odes_multigrid: the states-direction JVP (pea_jvp_odes with dU only) of every grid,
- * writing dR (and R when non-NULL). */
+ * compute_odes call, f90:496-537).  Results are bitwise those of one single-grid call per grid.
+ * pea_multi_prepare: uploads the launch's descriptor table built from the host array `grids`
+ *   (G entries) into `workspace` (device, pea_multi_workspace_bytes(G) bytes) and returns the
+ *   tile count; it synchronises the stream.  tangent = 0: residuals (R written); tangent = 1:
+ *   states-direction JVP (pea_jvp_odes with dU only; dR written, R too when non-NULL).
+ * pea_multi_launch: one launch over the prepared table; it allocates and copies nothing, so it
+ *   may be captured in a HIP graph and replayed while the table and arrays stay valid.
+ * pea_compute_odes_multi / pea_jvp_odes_multigrid: prepare + launch. */
 long long pea_multi_workspace_bytes(int G);
+int pea_multi_prepare(int G, const pea_grid *grids, int tangent, void *workspace,
+                      int *total_tiles, void *stream);
+int pea_multi_launch(int G, int total_tiles, int tangent, const void *workspace, void *stream);
 int pea_compute_odes_multi(int G, const pea_grid *grids, void *workspace, void *stream);
 int pea_jvp_odes_multigrid(int G, const pea_grid *grids, void *workspace, void *stream);
 

@@ -58,6 +58,8 @@ _SIGS = {
     "pea_export_bsr": [_I, _I, _P, _D, _D, _I, _P, _P, _P, _P],
     "pea_multi_workspace_bytes": [_I],
     "pea_compute_odes_multi": [_I, _P, _P, _P],
+    "pea_multi_prepare": [_I, _P, _I, _P, _P, _P],
+    "pea_multi_launch": [_I, _I, _I, _P, _P],
     "pea_jvp_odes_multigrid": [_I, _P, _P, _P],
     "pea_resolvent_last_error": [],
     "pea_block_multi": [_I, _I, _I, _P, _P, _I, _P, _P, _P],

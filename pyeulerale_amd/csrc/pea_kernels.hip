@@ -1752,10 +1752,10 @@ int pea_compute_odes_batch(const pea_params* prm, double mach, int M, int N, int
 
 long long pea_multi_workspace_bytes(int G) { return G < 1 ? 0 : (long long)G * sizeof(GridDesc); }
 
-static int multi_launch(int G, const pea_grid* grids, void* workspace, bool tangent,
-                        void* stream) {
+int pea_multi_prepare(int G, const pea_grid* grids, int tangent, void* workspace,
+                      int* total_tiles, void* stream) {
     if (G < 1) return fail(PEA_ERR_DIMS, "no grids");
-    if (!grids || !workspace) return fail(PEA_ERR_NULL, "null argument");
+    if (!grids || !workspace || !total_tiles) return fail(PEA_ERR_NULL, "null argument");
     const int tm = tangent ? kJvpTM : kFwdTM, tn = tangent ? kJvpTN : kFwdTN;
     std::vector<GridDesc> d(G);
     long long total = 0;
@@ -1770,38 +1770,55 @@ static int multi_launch(int G, const pea_grid* grids, void* workspace, bool tang
         total += (long long)d[i].tiles_m * ((q.N + tn - 1) / tn);
         if (total > 0x7fffffff) return fail(PEA_ERR_DIMS, "too many tiles");
     }
+    // synchronous upload: the host table may be released as soon as this returns
     hipStream_t st = as_stream(stream);
     if (hipMemcpyAsync(workspace, d.data(), G * sizeof(GridDesc), hipMemcpyHostToDevice, st) !=
-        hipSuccess)
-        return fail(PEA_ERR_HIP, "descriptor copy");
-    int dev = 0, cus = 0, per_cu = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-        return fail(PEA_ERR_HIP, "device query");
-    hipError_t oc = tangent
-        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_face_fwd_multi<true>,
-                                                       tm * tn, 0)
-        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_face_fwd_multi<false>,
-                                                       tm * tn, 0);
-    if (oc != hipSuccess || per_cu < 1) per_cu = 1;
-    long long nb = std::min<long long>(total, (long long)cus * per_cu);
+            hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return fail(PEA_ERR_HIP, "descriptor upload");
+    *total_tiles = (int)total;
+    return PEA_OK;
+}
+
+int pea_multi_launch(int G, int total_tiles, int tangent, const void* workspace, void* stream) {
+    if (G < 1 || total_tiles < 1) return fail(PEA_ERR_DIMS, "no tiles");
+    if (!workspace) return fail(PEA_ERR_NULL, "null argument");
+    const int nt = tangent ? kJvpTM * kJvpTN : kFwdTM * kFwdTN;
+    static int cus = 0, per_cu[2] = {0, 0};
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return fail(PEA_ERR_HIP, "device query");
+    }
+    int& pc = per_cu[tangent ? 1 : 0];
+    if (!pc) {
+        hipError_t oc = tangent
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, k_face_fwd_multi<true>, nt, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, k_face_fwd_multi<false>, nt, 0);
+        if (oc != hipSuccess || pc < 1) pc = 1;
+    }
+    long long nb = std::min<long long>(total_tiles, (long long)cus * pc);
     nb = std::max<long long>(kXcds, (nb + kXcds - 1) / kXcds * kXcds);
+    const GridDesc* d = static_cast<const GridDesc*>(workspace);
     if (tangent)
-        k_face_fwd_multi<true><<<(unsigned)nb, tm * tn, 0, st>>>(
-            static_cast<const GridDesc*>(workspace), G, (int)total);
+        k_face_fwd_multi<true><<<(unsigned)nb, nt, 0, as_stream(stream)>>>(d, G, total_tiles);
     else
-        k_face_fwd_multi<false><<<(unsigned)nb, tm * tn, 0, st>>>(
-            static_cast<const GridDesc*>(workspace), G, (int)total);
+        k_face_fwd_multi<false><<<(unsigned)nb, nt, 0, as_stream(stream)>>>(d, G, total_tiles);
     PEA_LAUNCH_CHECK("k_face_fwd_multi");
     return PEA_OK;
 }
 
 int pea_compute_odes_multi(int G, const pea_grid* grids, void* workspace, void* stream) {
-    return multi_launch(G, grids, workspace, false, stream);
+    int total = 0;
+    if (int rc = pea_multi_prepare(G, grids, 0, workspace, &total, stream)) return rc;
+    return pea_multi_launch(G, total, 0, workspace, stream);
 }
 
 int pea_jvp_odes_multigrid(int G, const pea_grid* grids, void* workspace, void* stream) {
-    return multi_launch(G, grids, workspace, true, stream);
+    int total = 0;
+    if (int rc = pea_multi_prepare(G, grids, 1, workspace, &total, stream)) return rc;
+    return pea_multi_launch(G, total, 1, workspace, stream);
 }
 
 int pea_compute_forces(int M, int N, const double* X, const double* U, double* F, double xc,

@@ -135,47 +135,59 @@ def compute_odes_batch(p: Params, mach: float, g: Grid, X, V, U, R=None) -> torc
     return R
 
 
-def _multi(items, tangent: bool):
-    grids = (_lib.PeaGrid * len(items))()
-    outs = []
-    for i, it in enumerate(items):
-        p, mach, X, V, U = it[:5]
-        g = Grid(U.shape[2], U.shape[1])
-        g.check_primal(X, V, U)
-        R = torch.empty(g.cell_shape, dtype=F64, device=U.device)
-        dU = dR = None
-        if tangent:
-            dU = _need(it[5], g.cell_shape, C128, "d_states")
-            dR = torch.empty(g.cell_shape, dtype=C128, device=U.device)
-        grids[i] = _lib.PeaGrid(g.M, g.N, float(mach), p.c(), X.data_ptr(), V.data_ptr(),
-                                U.data_ptr(), dU.data_ptr() if tangent else None, R.data_ptr(),
-                                dR.data_ptr() if tangent else None)
-        outs.append((R, dR) if tangent else R)
-    nbytes = int(_lib.load().pea_multi_workspace_bytes(len(items)))
-    ws = torch.empty(nbytes, dtype=torch.uint8, device=items[0][4].device)
-    # the descriptor table is copied into ws on the stream before the kernel reads it;
-    # keeping ws alive until the launch has run is the caller's stream order
-    _lib.call("pea_jvp_odes_multigrid" if tangent else "pea_compute_odes_multi", len(items),
-              grids, _ptr(ws), _stream())
-    _MULTI_KEEP.append(ws)
-    if len(_MULTI_KEEP) > 8:
-        del _MULTI_KEEP[0]
-    return outs
+class MultiGrid:
+    """A prepared multi-grid launch over many grids of uneven size.
 
+    ``items`` holds (params, mach, X, V, U) per grid in storage layout, plus dU for
+    ``tangent=True`` (the states-direction JVP).  The descriptor table is uploaded once;
+    :meth:`run` is then a single launch that allocates and copies nothing (it may be captured in
+    a CUDA graph) and returns the outputs: the list of R, or of (R, dR).  The input tensors must
+    stay alive and in place while the object is used."""
 
-_MULTI_KEEP: list = []
+    def __init__(self, items, tangent: bool = False):
+        self.tangent = bool(tangent)
+        self.items = list(items)
+        G = len(self.items)
+        if G < 1:
+            raise RuntimeError("no grids")
+        grids = (_lib.PeaGrid * G)()
+        self.outputs = []
+        for i, it in enumerate(self.items):
+            p, mach, X, V, U = it[:5]
+            g = Grid(U.shape[2], U.shape[1])
+            g.check_primal(X, V, U)
+            R = torch.empty(g.cell_shape, dtype=F64, device=U.device)
+            dU = dR = None
+            if self.tangent:
+                dU = _need(it[5], g.cell_shape, C128, "d_states")
+                dR = torch.empty(g.cell_shape, dtype=C128, device=U.device)
+            grids[i] = _lib.PeaGrid(g.M, g.N, float(mach), p.c(), X.data_ptr(), V.data_ptr(),
+                                    U.data_ptr(), dU.data_ptr() if dU is not None else None,
+                                    R.data_ptr(), dR.data_ptr() if dR is not None else None)
+            self.outputs.append((R, dR) if self.tangent else R)
+        nbytes = int(_lib.load().pea_multi_workspace_bytes(G))
+        self.ws = torch.empty(nbytes, dtype=torch.uint8, device=self.items[0][4].device)
+        total = ctypes.c_int(0)
+        _lib.call("pea_multi_prepare", G, grids, int(self.tangent), _ptr(self.ws),
+                  ctypes.byref(total), _stream())
+        self.G, self.total_tiles = G, total.value
+
+    def run(self):
+        _lib.call("pea_multi_launch", self.G, self.total_tiles, int(self.tangent), _ptr(self.ws),
+                  _stream())
+        return self.outputs
 
 
 def compute_odes_multi(items):
     """Residuals of many grids of uneven size in one persistent launch.  ``items`` is a list of
     (params, mach, X, V, U) in storage layout; returns the list of R."""
-    return _multi(items, False)
+    return MultiGrid(items).run()
 
 
 def jvp_odes_multigrid(items):
     """States-direction tangents of many grids in one launch: items (params, mach, X, V, U, dU);
     returns the list of (R, dR)."""
-    return _multi(items, True)
+    return MultiGrid(items, tangent=True).run()
 
 
 def compute_forces(g: Grid, X, U, F=None, center=(0.0, 0.0), coef=None):
