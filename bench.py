@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-freq", action="store_true", help="skip the frequency-response solve")
     ap.add_argument("--no-multigrid", action="store_true", help="skip the 64-grid launch")
+    ap.add_argument("--no-gmres", action="store_true", help="skip the 1025x513 GMRES steps")
     ap.add_argument("--no-superlu", action="store_true",
                     help="skip timing the reference's host SuperLU solve")
     ap.add_argument("--profile", action="store_true",
@@ -283,6 +284,37 @@ def multigrid(torch, K, G: int = 64, seed: int = 0, reps: int = 10):
             "bitwise_equal": bool(same)}
 
 
+def gmres_steps(torch, K, S: int = 32, steps: int = 30):
+    """Batched GMRES throughput at 1025x513 (where the exact ring LU does not fit one GPU):
+    30 Arnoldi steps (graph-replayed after the first cycle, radial-line preconditioner, block
+    operator) for 32 shifts s = i*logspace(1, 2, 32) at once, against the same steps for one
+    shift at a time.  Time per shift-step; convergence is not the point here."""
+    from pyeulerale_amd.gmres import BatchedGMRES
+    X, V, U, M, N = make_case(1025, 513, 0.04, 0.5, 0.0, 0.0, 0.0, 30.0, 1)
+    g, p = K.Grid(M, N), K.Params(aoa_deg=0.0)
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(np.swapaxes(a, -1, -2))).cuda()
+    Xd, Vd, Ud = dev(X), dev(V), dev(U[0])
+    JU, _, _, _ = K.cs_linearize_odes(p, 0.5, g, Xd, Vd, Ud, want=("states",))
+    shifts = 1j * np.logspace(1, 2, S)
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    b = torch.randn((S, *g.cell_shape), dtype=torch.complex128, device="cuda", generator=gen)
+
+    def run(sh, rhs):
+        gm = BatchedGMRES(g, JU, sh, restart=steps // 2, maxiter=steps, rtol=0.0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        gm.solve(rhs)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+    run(shifts[:1], b[:1])                         # library warm-up
+    t_b = run(shifts, b)
+    t_1 = run(shifts[:1], b[:1])
+    return {"grid": "1025x513", "shifts": S, "arnoldi_steps": steps,
+            "ms_per_shift_step_batched": t_b / (S * steps) * 1e3,
+            "ms_per_shift_step_single": t_1 / steps * 1e3,
+            "speedup_batched_vs_one_shift_at_a_time": (t_1 * S) / t_b}
+
+
 def cpu_baseline(seconds, X, V, U, mach, aoa):
     """The reference's compiled Fortran compute_odes on one host core (bounded sample)."""
     from oracle.oracle import Oracle, Params
@@ -353,6 +385,9 @@ def main():
         extra["products_1025x513"] = products(torch, K, 1025, 513, 0.04, 0.5, 0.0, 0.0, 0.0,
                                               30.0, 10)
         progress("products 1025x513 done")
+    if not args.no_gmres:
+        extra["gmres_1025x513"] = gmres_steps(torch, K)
+        progress("gmres done")
     if not args.no_multigrid:
         extra["multigrid_64_grids"] = multigrid(torch, K)
         progress("multi-grid done")

@@ -245,6 +245,12 @@ int pea_dense_lu_factor(int S, int n, double *A, int *ipiv, int *info, void *str
 int pea_dense_lu_solve(int S, int n, int nrhs, const double *A, const int *ipiv, double *B,
                        void *stream);
 
+/* GMRES: Givens update of column j of S Hessenberg matrices on the device (H (S, R+1, R), cs/sn
+ * (S, R), g (S, R+1) complex row-major; est[s] = |g[s][j+1]|), so that Arnoldi steps need no
+ * host round trip and can be captured in a HIP graph. */
+int pea_gmres_rotate(int S, int R, int j, double *H, double *cs, double *sn, double *g,
+                     double *est, void *stream);
+
 /* Block-Jacobi preconditioner: inverses of the 4x4 diagonal blocks (J_cc - s I);
  * F holds S*16*M*N complex values. */
 int pea_jacobi_factor(int M, int N, int S, const double *JU, const double *shifts,

@@ -504,6 +504,43 @@ k_ring_couple(int M, int N, int S, const double* __restrict__ JU, int transposed
 }
 
 
+// GMRES: Givens update of column j of the S Hessenberg matrices, in place (one thread per
+// shift).  H (S, R+1, R), cs and sn (S, R), g (S, R+1): complex, row-major; est (S): |g[j+1]|,
+// the residual norm of the least-squares problem after step j.  Rotations are
+// G = [[conj(c), conj(s)], [-s, c]] with c = a/|(a, b)|, s = b/|(a, b)|.
+__global__ void __launch_bounds__(64)
+k_gmres_rotate(int S, int R, int j, double* __restrict__ Hd, double* __restrict__ csd,
+               double* __restrict__ snd, double* __restrict__ gd, double* __restrict__ est) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    Z* H = reinterpret_cast<Z*>(Hd) + (long)s * (R + 1) * R;
+    Z* cs = reinterpret_cast<Z*>(csd) + (long)s * R;
+    Z* sn = reinterpret_cast<Z*>(snd) + (long)s * R;
+    Z* g = reinterpret_cast<Z*>(gd) + (long)s * (R + 1);
+    auto conj = [](Z a) { return Z{a.r, -a.i}; };
+    for (int i = 0; i < j; ++i) {
+        const Z hi = H[i * R + j], hi1 = H[(i + 1) * R + j];
+        H[i * R + j] = zadd(zmul(conj(cs[i]), hi), zmul(conj(sn[i]), hi1));
+        H[(i + 1) * R + j] = zsub(zmul(cs[i], hi1), zmul(sn[i], hi));
+    }
+    const Z a = H[j * R + j], b = H[(j + 1) * R + j];
+    const double den = sqrt(zabs2(a) + zabs2(b));
+    if (den == 0.0) {
+        cs[j] = Z{1.0, 0.0};
+        sn[j] = Z{0.0, 0.0};
+    } else {
+        cs[j] = Z{a.r / den, a.i / den};
+        sn[j] = Z{b.r / den, b.i / den};
+    }
+    H[j * R + j] = Z{den, 0.0};
+    H[(j + 1) * R + j] = Z{0.0, 0.0};
+    const Z gj = g[j];
+    g[j] = zmul(conj(cs[j]), gj);
+    const Z gn = zmul(sn[j], gj);
+    g[j + 1] = Z{-gn.r, -gn.i};
+    est[s] = sqrt(zabs2(g[j + 1]));
+}
+
 // One rocBLAS handle per device for the dense factorisations, created on first use.
 constexpr int kMaxDevices = 64;
 rocblas_handle g_handles[kMaxDevices] = {};
@@ -648,6 +685,14 @@ int pea_dense_lu_solve(int S, int n, int nrhs, const double* A, const int* ipiv,
         (rocblas_stride)n * n, ipiv, (rocblas_stride)n,
         reinterpret_cast<rocblas_double_complex*>(B), n, (rocblas_stride)n * nrhs, S);
     return st == rocblas_status_success ? PEA_OK : rfail(PEA_ERR_HIP, "rocsolver getrs");
+}
+
+int pea_gmres_rotate(int S, int R, int j, double* H, double* cs, double* sn, double* g,
+                     double* est, void* stream) {
+    if (S < 1 || R < 1 || j < 0 || j >= R) return rfail(PEA_ERR_DIMS, "bad dimensions");
+    if (!H || !cs || !sn || !g || !est) return rfail(PEA_ERR_NULL, "null argument");
+    k_gmres_rotate<<<(S + 63) / 64, 64, 0, (hipStream_t)stream>>>(S, R, j, H, cs, sn, g, est);
+    return hipGetLastError() == hipSuccess ? PEA_OK : rfail(PEA_ERR_HIP, "k_gmres_rotate");
 }
 
 }  // extern "C"

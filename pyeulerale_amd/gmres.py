@@ -9,7 +9,11 @@ The reference solves one shift at a time with SuperLU on a freshly assembled CSR
 * right preconditioning per shift: ``"line"`` (block LU of every radial line,
   ``pea_line_factor``/``pea_line_solve``), ``"jacobi"`` (4x4 diagonal blocks) or ``None``;
 * orthogonalisation: classical Gram-Schmidt applied twice (CGS2) as batched matrix products;
-* the small Hessenberg problems (Givens rotations, triangular solves) are done on the host.
+* the Hessenberg matrices stay on the device: ``pea_gmres_rotate`` applies the Givens rotations
+  of each step, one thread per shift, and the restart's triangular solves are batched.  An
+  Arnoldi step is thus device work only; the step for each index j is captured once as a CUDA
+  graph and replayed in later restart cycles (``graphs=True``).  The host reads the residual
+  estimates every ``check_every`` steps and the true residual once per cycle.
 
 The adjoint system (dR/dU - s I)^H y = c is (dR/dU)^T - conj(s) I, as core.py:594-598.
 """
@@ -48,7 +52,8 @@ class BatchedGMRES:
 
     def __init__(self, g: K.Grid, JU: torch.Tensor | None, shifts, *, adjoint: bool = False,
                  point=None, precond: str | None = "line", operator: str = "blocks",
-                 restart: int = 30, maxiter: int = 600, rtol: float = 1e-10):
+                 restart: int = 30, maxiter: int = 600, rtol: float = 1e-10,
+                 graphs: bool = True, check_every: int = 10):
         self.g = g
         self.adjoint = bool(adjoint)
         self.shifts = np.asarray(shifts, dtype=complex).reshape(-1)
@@ -59,6 +64,8 @@ class BatchedGMRES:
         self.restart = int(restart)
         self.maxiter = int(maxiter)
         self.rtol = float(rtol)
+        self.graphs = bool(graphs)
+        self.check_every = max(1, int(check_every))
         dev = JU.device if JU is not None else point[5].device
         self.device = dev
         self.sh = torch.as_tensor(self.shifts, dtype=C128, device=dev).contiguous()
@@ -120,99 +127,114 @@ class BatchedGMRES:
         return z
 
     # ------------------------------------------------------------------ solve
+    def _step(self, j, st):
+        """Arnoldi step j for all shifts, entirely on the device: w = A P^-1 v_j, CGS2 against
+        v_0..v_j, v_{j+1}, column j of H, and its Givens update (``pea_gmres_rotate``)."""
+        S, n, R = self.S, st["n"], self.restart
+        V, w, z = st["V"], st["w"], st["z"]
+        w.copy_(V[:, j].reshape(w.shape))
+        self.psolve(w, z)
+        self.apply(z, w)
+        wf = w.reshape(S, n, 1)
+        Vj = V[:, :j + 1]
+        # CGS2: two passes of h = V^H w, w -= V h (the basis itself is never conjugated or
+        # copied: h = conj(V conj(w)))
+        h = torch.bmm(Vj, wf.conj().resolve_conj()).conj().resolve_conj()
+        wf -= torch.bmm(h.transpose(1, 2).contiguous(), Vj).transpose(1, 2)
+        h2 = torch.bmm(Vj, wf.conj().resolve_conj()).conj().resolve_conj()
+        wf -= torch.bmm(h2.transpose(1, 2).contiguous(), Vj).transpose(1, 2)
+        h = h + h2
+        hn = torch.linalg.vector_norm(wf[:, :, 0], dim=1)
+        V[:, j + 1] = wf[:, :, 0] / hn.clamp_min(1e-300)[:, None]
+        H = st["H"]
+        H[:, :j + 1, j] = h[:, :, 0]
+        H[:, j + 1, j] = hn.to(C128)
+        _lib.call("pea_gmres_rotate", S, R, j, _p(H), _p(st["cs"]), _p(st["sn"]), _p(st["g"]),
+                  _p(st["est"]), K._stream())
+        st["hist"][j].copy_(st["est"])
+
+    def _capture(self, st):
+        """One CUDA graph per step index j, captured once (after an eager cycle has warmed the
+        libraries up) and replayed in every later restart cycle."""
+        graphs = []
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        for j in range(self.restart):
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=side):
+                self._step(j, st)
+            graphs.append(g)
+        torch.cuda.current_stream().wait_stream(side)
+        return graphs
+
     def solve(self, b: torch.Tensor, x0: torch.Tensor | None = None) -> torch.Tensor:
         """Solves for all shifts; b of shape (S, 4, N, M) complex (canonical storage)."""
         t0 = time.perf_counter()
         S, R = self.S, self.restart
         shape = (S, *self.g.cell_shape)
+        dev = self.device
         b = b.to(C128).reshape(shape).contiguous()
         n = b[0].numel()
         x = torch.zeros_like(b) if x0 is None else x0.to(C128).reshape(shape).clone()
-        V = torch.empty((S, R + 1, n), dtype=C128, device=self.device)
-        w = torch.empty(shape, dtype=C128, device=self.device)
-        z = torch.empty(shape, dtype=C128, device=self.device)
+        st = dict(n=n, V=torch.empty((S, R + 1, n), dtype=C128, device=dev),
+                  w=torch.empty(shape, dtype=C128, device=dev),
+                  z=torch.empty(shape, dtype=C128, device=dev),
+                  H=torch.zeros((S, R + 1, R), dtype=C128, device=dev),
+                  cs=torch.zeros((S, R), dtype=C128, device=dev),
+                  sn=torch.zeros((S, R), dtype=C128, device=dev),
+                  g=torch.zeros((S, R + 1), dtype=C128, device=dev),
+                  est=torch.zeros(S, dtype=torch.float64, device=dev),
+                  hist=torch.zeros((R, S), dtype=torch.float64, device=dev))
+        w = st["w"]
         bnorm = torch.linalg.vector_norm(b.reshape(S, n), dim=1).cpu().numpy()
         bnorm[bnorm == 0] = 1.0
+        bnorm_d = torch.as_tensor(bnorm, device=dev)
         iters = np.zeros(S, dtype=int)
-        resid = np.ones(S)
-        converged = np.zeros(S, dtype=bool)
         history = []
-        total = 0
-        restarts = 0
+        total = restarts = 0
+        graphs = None
         while True:
             # r = b - A x; the true residual decides convergence, also after the last cycle
             self.apply(x, w)
             r = b - w
             beta = torch.linalg.vector_norm(r.reshape(S, n), dim=1)
-            beta_h = beta.cpu().numpy()
-            resid = beta_h / bnorm
+            resid = beta.cpu().numpy() / bnorm
             converged = resid <= self.rtol
             if converged.all() or total >= self.maxiter:
                 break
-            V[:, 0] = (r.reshape(S, n) / beta.clamp_min(1e-300)[:, None])
-            H = np.zeros((S, R + 1, R), dtype=complex)
-            cs = np.zeros((S, R), dtype=complex)
-            sn = np.zeros((S, R), dtype=complex)
-            gvec = np.zeros((S, R + 1), dtype=complex)
-            gvec[:, 0] = beta_h
+            st["V"][:, 0] = r.reshape(S, n) / beta.clamp_min(1e-300)[:, None]
+            for key in ("H", "cs", "sn", "g"):
+                st[key].zero_()     # fill kernels: these buffers may also be graph inputs
+            st["g"][:, 0] = beta.to(C128)
             active = ~converged
             j_done = 0
+            use_graphs = self.graphs and restarts > 0
+            if use_graphs and graphs is None:
+                graphs = self._capture(st)
             for j in range(R):
-                w.copy_(V[:, j].reshape(shape))
-                self.psolve(w, z)
-                self.apply(z, w)
-                wf = w.reshape(S, n, 1)
-                Vj = V[:, :j + 1]
-                # CGS2: two passes of h = V^H w, w -= V h (the basis itself is never
-                # conjugated or copied: h = conj(V conj(w)))
-                h = torch.bmm(Vj, wf.conj().resolve_conj()).conj().resolve_conj()
-                wf -= torch.bmm(h.transpose(1, 2).contiguous(), Vj).transpose(1, 2)
-                h2 = torch.bmm(Vj, wf.conj().resolve_conj()).conj().resolve_conj()
-                wf -= torch.bmm(h2.transpose(1, 2).contiguous(), Vj).transpose(1, 2)
-                h = h + h2
-                hn = torch.linalg.vector_norm(wf[:, :, 0], dim=1)
-                V[:, j + 1] = wf[:, :, 0] / hn.clamp_min(1e-300)[:, None]
-                hc = h[:, :, 0].cpu().numpy()
-                hnh = hn.cpu().numpy()
-                H[:, :j + 1, j] = hc
-                H[:, j + 1, j] = hnh
-                # complex Givens rotations G = [[conj(c), conj(s)], [-s, c]], per shift
-                for s in range(S):
-                    for i in range(j):
-                        hi, hi1 = H[s, i, j], H[s, i + 1, j]
-                        H[s, i, j] = np.conj(cs[s, i]) * hi + np.conj(sn[s, i]) * hi1
-                        H[s, i + 1, j] = -sn[s, i] * hi + cs[s, i] * hi1
-                    a, bb = H[s, j, j], H[s, j + 1, j]
-                    den = np.sqrt(abs(a) ** 2 + abs(bb) ** 2)
-                    if den == 0.0:
-                        cs[s, j], sn[s, j] = 1.0, 0.0
-                    else:
-                        cs[s, j] = a / den
-                        sn[s, j] = bb / den
-                    H[s, j, j] = den
-                    H[s, j + 1, j] = 0.0
-                    gj = gvec[s, j]
-                    gvec[s, j] = np.conj(cs[s, j]) * gj
-                    gvec[s, j + 1] = -sn[s, j] * gj
+                if use_graphs:
+                    graphs[j].replay()
+                else:
+                    self._step(j, st)
                 j_done = j + 1
                 total += 1
-                est = np.abs(gvec[:, j + 1]) / bnorm
+                if j_done % self.check_every == 0 or j_done == R or total >= self.maxiter:
+                    est = (st["est"] / bnorm_d).cpu().numpy()
+                    if not (active & (est > self.rtol)).any() or total >= self.maxiter:
+                        break
+            hist = (st["hist"][:j_done] / bnorm_d).cpu().numpy()
+            for jj in range(j_done):
                 iters[active] += 1
-                active &= est > self.rtol
-                history.append(est.copy())
-                if not active.any() or total >= self.maxiter:
-                    break
-            # x += P^-1 V y
-            Y = np.zeros((S, j_done), dtype=complex)
-            for s in range(S):
-                if converged[s]:
-                    continue
-                Hs = H[s, :j_done, :j_done]
-                Y[s] = np.linalg.solve(np.triu(Hs), gvec[s, :j_done]) if j_done else Y[s]
-            Yd = torch.as_tensor(Y, dtype=C128, device=self.device)
-            upd = torch.bmm(Yd[:, None, :], V[:, :j_done]).reshape(shape)
-            self.psolve(upd.contiguous(), z)
-            x += z
+                active &= hist[jj] > self.rtol
+                history.append(hist[jj].copy())
+            # x += P^-1 V y, y = H[:j, :j]^-1 g[:j] (upper triangular), zero for shifts that had
+            # converged before the cycle
+            Hj = torch.triu(st["H"][:, :j_done, :j_done])
+            Y = torch.linalg.solve_triangular(Hj, st["g"][:, :j_done, None], upper=True)
+            Y[torch.as_tensor(converged, device=dev)] = 0
+            upd = torch.bmm(Y.transpose(1, 2).contiguous(), st["V"][:, :j_done]).reshape(shape)
+            self.psolve(upd.contiguous(), st["z"])
+            x += st["z"]
             restarts += 1
         self.info = GMRESInfo(iterations=iters, residuals=resid, converged=converged,
                               restarts=restarts, seconds=time.perf_counter() - t0,

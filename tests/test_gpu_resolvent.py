@@ -122,3 +122,24 @@ def test_matrix_free_operator_and_batched_equals_sequential(sd):
     for k, s in enumerate(shifts):
         xs = sd.solve_odes_wrt_states_fwd(b[k], shift=s, solver="gmres", rtol=1e-12)
         assert rel(xs.cpu().numpy(), xb[k]) <= 1e-9
+
+
+def test_gmres_graph_replays_match_eager(sd):
+    """The captured Arnoldi steps, replayed over many restart cycles (restart 8, with eager
+    residual and update work and host reads between replays), reproduce the uncaptured solver."""
+    from pyeulerale_amd.gmres import BatchedGMRES
+    rng = np.random.default_rng(5)
+    shifts = [complex(0, 10.0), complex(3.0, 0.0), complex(5.0, 5.0)]
+    b = torch.as_tensor(crandn(rng, (3, 4, sd.num_angular, sd.num_radial))).cuda()
+    out = {}
+    for graphs in (False, True):
+        gm = BatchedGMRES.from_discretization(sd, shifts, restart=8, maxiter=400, rtol=1e-12,
+                                              graphs=graphs, check_every=4)
+        print(f"[resolvent] gmres graphs={graphs}", flush=True)
+        out[graphs] = (gm.solve(b).cpu().numpy(), gm.info)
+    xe, ie = out[False]
+    xg, ig = out[True]
+    assert ie.converged.all() and ig.converged.all(), (ie.residuals, ig.residuals)
+    assert ig.restarts >= 3
+    assert (ie.iterations == ig.iterations).all(), (ie.iterations, ig.iterations)
+    assert rel(xg, xe) <= 1e-13, rel(xg, xe)
