@@ -251,6 +251,13 @@ int pea_dense_lu_solve(int S, int n, int nrhs, const double *A, const int *ipiv,
 int pea_gmres_rotate(int S, int R, int j, double *H, double *cs, double *sn, double *g,
                      double *est, void *stream);
 
+/* Gram-Schmidt projections: h[s*bsh + i] = sum_k conj(V[s*bsv + i*ldv + k]) w[s*bsw + k] for
+ * i < q, k < n, s < S (complex; strides in complex elements).  K is split into nb chunks
+ * summed in a fixed order; P is a workspace of S*nb*q complex values. */
+int pea_multidot(int S, int n, int q, const double *V, long long ldv, long long bsv,
+                 const double *w, long long bsw, int nb, double *P, double *h, long long bsh,
+                 void *stream);
+
 /* Block-Jacobi preconditioner: inverses of the 4x4 diagonal blocks (J_cc - s I);
  * F holds S*16*M*N complex values. */
 int pea_jacobi_factor(int M, int N, int S, const double *JU, const double *shifts,

@@ -541,6 +541,64 @@ k_gmres_rotate(int S, int R, int j, double* __restrict__ Hd, double* __restrict_
     est[s] = sqrt(zabs2(g[j + 1]));
 }
 
+// Batched multi-dot for Gram-Schmidt: h[s][i] = sum_k conj(V[s][i][k]) w[s][k] for i < q, with
+// V rows of length n (row stride ldv, batch stride bsv) and w of batch stride bsw.  Library
+// GEMMs handle this shape (tiny output, K of millions) without splitting K; here stage 1 cuts K
+// into nb chunks, one workgroup per (chunk, s), and stage 2 adds the chunks in a fixed order:
+// deterministic, no atomics, no memset.
+constexpr int kDotThreads = 256;
+
+__global__ void __launch_bounds__(kDotThreads)
+k_multidot_partial(int n, int q, const double* __restrict__ V, long ldv, long bsv,
+                   const double* __restrict__ w, long bsw, int nb, double* __restrict__ P) {
+    const int b = blockIdx.x, s = blockIdx.y, tid = threadIdx.x;
+    const long chunk = ((long)n + nb - 1) / nb;
+    const long k0 = (long)b * chunk, k1 = min((long)n, k0 + chunk);
+    __shared__ double red_r[kDotThreads], red_i[kDotThreads];
+    const double* ws = w + 2 * (long)s * bsw;
+    for (int i = 0; i < q; ++i) {
+        const double* v = V + 2 * ((long)s * bsv + (long)i * ldv);
+        double ar = 0.0, ai = 0.0;
+        for (long k = k0 + tid; k < k1; k += kDotThreads) {
+            const double vr = v[2 * k], vi = v[2 * k + 1], xr = ws[2 * k], xi = ws[2 * k + 1];
+            ar = fma(vr, xr, fma(vi, xi, ar));    // conj(v) * x
+            ai = fma(vr, xi, fma(-vi, xr, ai));
+        }
+        red_r[tid] = ar;
+        red_i[tid] = ai;
+        __syncthreads();
+        for (int h = kDotThreads / 2; h > 0; h >>= 1) {
+            if (tid < h) {
+                red_r[tid] += red_r[tid + h];
+                red_i[tid] += red_i[tid + h];
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            const long e = ((long)s * nb + b) * q + i;
+            P[2 * e] = red_r[0];
+            P[2 * e + 1] = red_i[0];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(64)
+k_multidot_final(int S, int q, int nb, const double* __restrict__ P, double* __restrict__ h,
+                 long bsh) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= S * q) return;
+    const int s = t / q, i = t - s * q;
+    double ar = 0.0, ai = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        const long e = ((long)s * nb + b) * q + i;
+        ar += P[2 * e];
+        ai += P[2 * e + 1];
+    }
+    h[2 * ((long)s * bsh + i)] = ar;
+    h[2 * ((long)s * bsh + i) + 1] = ai;
+}
+
 // One rocBLAS handle per device for the dense factorisations, created on first use.
 constexpr int kMaxDevices = 64;
 rocblas_handle g_handles[kMaxDevices] = {};
@@ -693,6 +751,17 @@ int pea_gmres_rotate(int S, int R, int j, double* H, double* cs, double* sn, dou
     if (!H || !cs || !sn || !g || !est) return rfail(PEA_ERR_NULL, "null argument");
     k_gmres_rotate<<<(S + 63) / 64, 64, 0, (hipStream_t)stream>>>(S, R, j, H, cs, sn, g, est);
     return hipGetLastError() == hipSuccess ? PEA_OK : rfail(PEA_ERR_HIP, "k_gmres_rotate");
+}
+
+int pea_multidot(int S, int n, int q, const double* V, long long ldv, long long bsv,
+                 const double* w, long long bsw, int nb, double* P, double* h, long long bsh,
+                 void* stream) {
+    if (S < 1 || n < 1 || q < 1 || nb < 1 || S > 65535) return rfail(PEA_ERR_DIMS, "bad dimensions");
+    if (!V || !w || !P || !h) return rfail(PEA_ERR_NULL, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    k_multidot_partial<<<dim3(nb, S), kDotThreads, 0, st>>>(n, q, V, ldv, bsv, w, bsw, nb, P);
+    k_multidot_final<<<(S * q + 63) / 64, 64, 0, st>>>(S, q, nb, P, h, bsh);
+    return hipGetLastError() == hipSuccess ? PEA_OK : rfail(PEA_ERR_HIP, "k_multidot");
 }
 
 }  // extern "C"

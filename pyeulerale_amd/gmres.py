@@ -137,14 +137,17 @@ class BatchedGMRES:
         self.apply(z, w)
         wf = w.reshape(S, n, 1)
         Vj = V[:, :j + 1]
-        # CGS2: two passes of h = V^H w, w -= V h (the basis itself is never conjugated or
-        # copied: h = conj(V conj(w)))
-        h = torch.bmm(Vj, wf.conj().resolve_conj()).conj().resolve_conj()
-        wf -= torch.bmm(h.transpose(1, 2).contiguous(), Vj).transpose(1, 2)
-        h2 = torch.bmm(Vj, wf.conj().resolve_conj()).conj().resolve_conj()
-        wf -= torch.bmm(h2.transpose(1, 2).contiguous(), Vj).transpose(1, 2)
-        h = h + h2
-        hn = torch.linalg.vector_norm(wf[:, :, 0], dim=1)
+        q = j + 1
+        # CGS2: two passes of h = V^H w, w -= V h.  The projections use pea_multidot (K split
+        # into chunks); a library GEMM of this shape (<= 31 x 1 output, K = 4C) does not split K.
+        h1, h2 = st["h1"], st["h2"]
+        self._dots(st, V, R + 1, q, w, h1)
+        wf -= torch.bmm(h1[:, None, :q], Vj).transpose(1, 2)
+        self._dots(st, V, R + 1, q, w, h2)
+        wf -= torch.bmm(h2[:, None, :q], Vj).transpose(1, 2)
+        h = (h1[:, :q] + h2[:, :q])[:, :, None]
+        self._dots(st, w, 1, 1, w, st["nrm"])
+        hn = st["nrm"][:, 0].real.sqrt()
         V[:, j + 1] = wf[:, :, 0] / hn.clamp_min(1e-300)[:, None]
         H = st["H"]
         H[:, :j + 1, j] = h[:, :, 0]
@@ -152,6 +155,12 @@ class BatchedGMRES:
         _lib.call("pea_gmres_rotate", S, R, j, _p(H), _p(st["cs"]), _p(st["sn"]), _p(st["g"]),
                   _p(st["est"]), K._stream())
         st["hist"][j].copy_(st["est"])
+
+    def _dots(self, st, V, rows, q, w, out):
+        """out[s, i] = sum_k conj(V[s, i, k]) w[s, k] for i < q (V with ``rows`` rows per shift)."""
+        n = st["n"]
+        _lib.call("pea_multidot", self.S, n, q, _p(V), n, rows * n, _p(w), n, st["nb"],
+                  _p(st["P"]), _p(out), out.shape[1], K._stream())
 
     def _capture(self, st):
         """One CUDA graph per step index j, captured once (after an eager cycle has warmed the
@@ -184,7 +193,12 @@ class BatchedGMRES:
                   sn=torch.zeros((S, R), dtype=C128, device=dev),
                   g=torch.zeros((S, R + 1), dtype=C128, device=dev),
                   est=torch.zeros(S, dtype=torch.float64, device=dev),
-                  hist=torch.zeros((R, S), dtype=torch.float64, device=dev))
+                  hist=torch.zeros((R, S), dtype=torch.float64, device=dev),
+                  h1=torch.zeros((S, R + 1), dtype=C128, device=dev),
+                  h2=torch.zeros((S, R + 1), dtype=C128, device=dev),
+                  nrm=torch.zeros((S, 1), dtype=C128, device=dev),
+                  nb=max(8, min(1024, 2048 // S)))
+        st["P"] = torch.empty((S, st["nb"], R + 1), dtype=C128, device=dev)
         w = st["w"]
         bnorm = torch.linalg.vector_norm(b.reshape(S, n), dim=1).cpu().numpy()
         bnorm[bnorm == 0] = 1.0
