@@ -63,6 +63,15 @@ static Grid make_grid(int M, int N) { return Grid{M, N, M * N, (M + 1) * (N + 1)
 
 PEA_HD int wrapn(int n, int N) { return n < 0 ? n + N : (n >= N ? n - N : n); }
 
+// MI355X dispatches workgroup b to XCD b mod 8, and each XCD has its own L2.  Tiles that share
+// halo cells should run on one XCD: xcd_tile() maps the linear workgroup id to a tile id so that
+// XCD x walks the contiguous tile range [start(x), start(x+1)) (a bijection for any total).
+constexpr int kXcdCount = 8;
+__device__ __forceinline__ long xcd_tile(long b, long total) {
+    const long q = total / kXcdCount, r = total % kXcdCount, x = b % kXcdCount;
+    return x * q + (x < r ? x : r) + b / kXcdCount;
+}
+
 // cell index (n*M + m) of stencil slot `slot` of cell (m, n); radial neighbours are only
 // requested where they exist (boundary flags)
 PEA_HD int slot_cell(const Grid& g, int m, int n, int slot) {
@@ -526,7 +535,10 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const Ld& ld,
 template <class T, class G, int TM, int TN, class Ld, class Store>
 __global__ void __launch_bounds__(TM * TN)
 k_face_fwd(pea_params P, Ld ld, Store store) {
-    face_fwd_tile<T, G, TM, TN>(P, ld, store, blockIdx.x, blockIdx.y, blockIdx.z);
+    const long nx = gridDim.x, ny = gridDim.y;
+    const long t = xcd_tile(blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z), nx * ny * gridDim.z);
+    face_fwd_tile<T, G, TM, TN>(P, ld, store, (int)(t % nx), (int)(t / nx % ny),
+                                (int)(t / (nx * ny)));
 }
 
 struct StoreRealB {
@@ -1194,7 +1206,9 @@ k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __
     __shared__ T accU[4][L][NC], accP[L][NC], accC[L][NC];
     __shared__ T gU_[2][TN][4][L], gP_[2][TN][L];   // ghost adjoints of the tile's columns
     const Grid g = ld.g;
-    const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
+    const long tile = xcd_tile(blockIdx.x + (long)gridDim.x * blockIdx.y,
+                               (long)gridDim.x * gridDim.y);   // XCD-contiguous tiles
+    const int m0 = (int)(tile % gridDim.x) * TM, n0 = (int)(tile / gridDim.x) * TN;
     const int tid = threadIdx.x;
 
     auto vtx = [&](int mm, int nn, T* x) { ld.vertex(nn * (g.M + 1) + mm, x); };
