@@ -1527,8 +1527,18 @@ unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 int cell_block(long C) { return C >= 256L * 1024 ? 256 : 64; }
 
 // tiles of the face-based forward kernels (m fastest within a tile)
-constexpr int kFwdTM = 32, kFwdTN = 8;
-constexpr int kJvpTM = 32, kJvpTN = 4;
+// Tile shapes of the forward kernels (cells per workgroup, one thread per cell); the defaults
+// are the fastest measured on MI355X and may be overridden at build time.
+#ifndef PEA_FWD_TILE_M
+#define PEA_FWD_TILE_M 32
+#define PEA_FWD_TILE_N 8
+#endif
+#ifndef PEA_JVP_TILE_M
+#define PEA_JVP_TILE_M 32
+#define PEA_JVP_TILE_N 4
+#endif
+constexpr int kFwdTM = PEA_FWD_TILE_M, kFwdTN = PEA_FWD_TILE_N;
+constexpr int kJvpTM = PEA_JVP_TILE_M, kJvpTN = PEA_JVP_TILE_N;
 
 // ------------------------------------------------------------------ many grids, one launch
 // Descriptor of one grid of a multi-grid launch as the kernel reads it: the public pea_grid

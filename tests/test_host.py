@@ -106,3 +106,22 @@ def test_python_layer_checks_shapes_before_launch():
     with pytest.raises(RuntimeError):
         K.compute_odes(K.Params(), 0.5, g, torch.zeros(g.vert_shape, dtype=torch.float64),
                        torch.zeros(g.vert_shape, dtype=torch.float64), bad)
+
+
+def test_example_fixtures_parse_and_grids_read():
+    """The reference's example grids (gzipped PLOT3D) and CI-pinned outputs copied into
+    tests/golden/examples/ are readable: the grid shapes of the 129x129 files, and the PTC /
+    frequency-response / gain lines of every .out file."""
+    from pathlib import Path
+    from pyeulerale_amd.grid import read_plot3d
+    from test_gpu_examples import parse_out
+    gold = Path(__file__).parent / "golden" / "examples"
+    for name in ("naca0004_129x129.plot3d.gz", "naca0012_129x129.plot3d.gz"):
+        X = read_plot3d(gold / name)
+        assert X.shape == (2, 129, 129)
+        np.testing.assert_array_equal(X[:, :, 0], X[:, :, -1])   # closed O-grid (App. C)
+    ptc, freq, gains = parse_out(gold / "response_Ma0p5.out")
+    assert len(ptc) == 12 and len(freq) == 11 and not gains
+    assert freq[0][1] == complex("+7.386e+00-3.482e-01j")
+    ptc, freq, gains = parse_out(gold / "gain_Ma0p50.out")
+    assert len(ptc) == 11 and not freq and gains == [8.2518, 7.1469]
