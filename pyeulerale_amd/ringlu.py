@@ -55,7 +55,7 @@ class RingLU:
     """Factorisation of (dR/dU - s_k I), or of the adjoint system, for the shifts s_k."""
 
     def __init__(self, g: K.Grid, JU: torch.Tensor, shifts, *, adjoint: bool = False,
-                 refine: int = 1):
+                 refine: int = 1, dense: str = "auto"):
         if JU.device.type != "cuda":
             raise RuntimeError("RingLU runs on the GPU (pea_ring_band / pea_ring_couple)")
         K._need(JU, (9, 4, 4, g.N, g.M), torch.float64, "JU")
@@ -65,6 +65,11 @@ class RingLU:
         self.shifts = np.asarray(shifts, dtype=complex).reshape(-1)
         self.S = len(self.shifts)
         self.refine = int(refine)
+        # dense LU of the pivot blocks: rocSOLVER strided-batched for several shifts; for one
+        # shift torch's single-matrix LU (faster than a batch of one), same factors
+        if dense not in ("auto", "rocsolver", "torch"):
+            raise ValueError(f"unknown dense LU backend {dense!r}")
+        self.dense = ("torch" if self.S == 1 else "rocsolver") if dense == "auto" else dense
         self.device = JU.device
         self.sh = torch.as_tensor(self.shifts, dtype=C128, device=self.device).contiguous()
         self._factor()
@@ -94,7 +99,10 @@ class RingLU:
         return D
 
     def _lu_solve(self, k, B):
-        """B <- P_k^-1 B in place; B column-major storage (S, q, n)."""
+        """P_k^-1 B for B in column-major storage (S, q, n); returns the same storage (B itself
+        with rocSOLVER)."""
+        if self.dense == "torch":
+            return torch.linalg.lu_solve(self.LU[k], self.piv[k], B.mT).mT.contiguous()
         n = 4 * self.g.N
         _lib.call("pea_dense_lu_solve", self.S, n, B.shape[1], _p(self.LU[k]), _p(self.piv[k]),
                   _p(B), K._stream())
@@ -118,9 +126,15 @@ class RingLU:
                 if R1 is not None:
                     R1.mT.baddbmm_(B.mT, self.W2[k - 1].mT, alpha=-1)
                 del B
-            piv = torch.empty((S, n), dtype=torch.int32, device=self.device)
-            _lib.call("pea_dense_lu_factor", S, n, _p(P), _p(piv), _p(info[k]), K._stream())
-            self.LU[k], self.piv[k] = P, piv
+            if self.dense == "torch":
+                LU, piv, inf = torch.linalg.lu_factor_ex(P.mT)
+                info[k].copy_(inf)
+                del P
+            else:
+                LU = P
+                piv = torch.empty((S, n), dtype=torch.int32, device=self.device)
+                _lib.call("pea_dense_lu_factor", S, n, _p(P), _p(piv), _p(info[k]), K._stream())
+            self.LU[k], self.piv[k] = LU, piv
             if R1 is not None:
                 self.W1[k] = self._lu_solve(k, R1)
             if k + 2 < M:
