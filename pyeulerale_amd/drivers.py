@@ -11,6 +11,7 @@ example scripts with all frequencies solved in one batch.
 
 from __future__ import annotations
 
+import gzip
 import math
 from typing import Callable
 
@@ -63,14 +64,41 @@ def pitch_gain(sd, axis_location: float) -> torch.Tensor:
     return torch.as_tensor(rotation_gain(X, (axis_location, 0.0))).to(C128).to(sd.device)
 
 
+def export_pressure_coefficients(file_name, vertices, pressure_coef, pressure_coef_wrt_aoa) -> None:
+    """Writes c_p and its pitch transfer for plotting, gzipped, as the response example does
+    (examples/response/run.py:32-64).  Each cell (m, n), in row-major order of (M, N), gets four
+    lines ``x z c_p Re(L c_p/L alpha) Im(L c_p/L alpha)`` (its vertices bi, fi, fo, bo) and an
+    empty line.  ``vertices`` (2, M+1, N+1), the coefficients (M, N); arrays or tensors."""
+    as_np = lambda a: a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)
+    X = as_np(vertices).real
+    cp = as_np(pressure_coef).real
+    dcp = as_np(pressure_coef_wrt_aoa)
+    M, N = cp.shape
+    corners = np.stack([X[:, :-1, :-1], X[:, :-1, 1:], X[:, 1:, 1:], X[:, 1:, :-1]], axis=-1)
+    rows = np.empty((M, N, 4, 5))
+    rows[..., 0] = corners[0]
+    rows[..., 1] = corners[1]
+    rows[..., 2] = cp[..., None]
+    rows[..., 3] = dcp.real[..., None]
+    rows[..., 4] = dcp.imag[..., None]
+    lines = []
+    for cell in rows.reshape(M * N, 4, 5):
+        lines.extend(" ".join(f"{v:+.5e}" for v in row) for row in cell)
+        lines.append("")
+    with gzip.open(file_name, "wt") as fh:
+        fh.write("\n".join(lines) + "\n")
+
+
 def frequency_response(sd, reduced_frequencies, chord: float, axis_location: float,
                        batched: bool = True, solver: str = "direct",
-                       log: Callable | None = print, **solve_opts):
+                       log: Callable | None = print, cp_file=None, **solve_opts):
     """Transfer functions from pitch angle to c_l and c_m at the linearization point.
 
     ``sd`` must be linearized at a steady state and constructed with
     ``coefficient_length=chord`` and ``coefficient_center=(axis_location, 0)``.
     Returns (reduced frequencies, L c_l / L alpha, L c_m / L alpha) as complex arrays.
+    With ``cp_file`` the steady c_p and its transfer at the last frequency are exported
+    (:func:`export_pressure_coefficients`), as the example does.
 
     ``batched=True`` solves all shifts together: ``solver="direct"`` factorises them in one
     batched ring LU, ``solver="gmres"`` runs batched GMRES and solves the shifts it does not
@@ -100,6 +128,10 @@ def frequency_response(sd, reduced_frequencies, chord: float, axis_location: flo
         cl.append(complex(torch.vdot(gl, dF)))
         cm.append(complex(torch.vdot(gm, dF) + torch.vdot(gmx, dX.reshape(-1))))
     cl, cm = np.array(cl), np.array(cm)
+    if cp_file is not None:
+        export_pressure_coefficients(
+            cp_file, sd.vertices, sd.compute_pressure_coefficients(),
+            sd.apply_pressure_coefficients_wrt_states_fwd(dU[-1]))
     if log:
         log(f"\nFrequency response:\n"
             f"{'red frequency':>15} {'lift coefficient':>25} {'moment coefficient':>25}")

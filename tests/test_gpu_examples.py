@@ -60,7 +60,14 @@ def close_printed(a: complex, b: complex, digits: int = 4) -> bool:
     return ok(a.real, b.real) and ok(a.imag, b.imag)
 
 
-def _response(mach: float):
+def _read_cp(path):
+    import gzip
+    with gzip.open(path, "rt") as fh:
+        lines = [ln for ln in fh.read().splitlines() if ln.strip()]
+    return lines, np.array([[float(t) for t in ln.split()] for ln in lines])
+
+
+def _response(mach: float, cp_file=None):
     from pyeulerale_amd.core import SpatialDiscretization
     from pyeulerale_amd.drivers import frequency_response, pseudo_transient_continuation
     sd = SpatialDiscretization(grid_file=GOLD / "naca0004_129x129.plot3d.gz",
@@ -70,15 +77,16 @@ def _response(mach: float):
     log = lambda s: (lines.extend(s.splitlines()), print(s, flush=True))
     pseudo_transient_continuation(sd, rtol=1e-9, max_iter=100, log=log)
     k, cl, cm = frequency_response(sd, np.logspace(-2, 0, 11), chord=10.0, axis_location=0.0,
-                                   log=log)
+                                   log=log, cp_file=cp_file)
     return sd, lines, k, cl, cm
 
 
 @pytest.mark.parametrize("case,mach", [("Ma0p05", 0.05), ("Ma0p5", 0.5)])
-def test_response_example(case, mach):
+def test_response_example(case, mach, tmp_path):
     print(f"[examples] response {case}", flush=True)
     ref_ptc, ref_freq, _ = parse_out(GOLD / f"response_{case}.out")
-    sd, lines, k, cl, cm = _response(mach)
+    cp_file = tmp_path / "cp.gz" if mach == 0.5 else None
+    sd, lines, k, cl, cm = _response(mach, cp_file)
     check_ptc(lines, ref_ptc)
     assert len(ref_freq) == len(k) == 11
     exact = 0
@@ -87,6 +95,20 @@ def test_response_example(case, mach):
         assert close_printed(cl[i], clr) and close_printed(cm[i], cmr), (i, cl[i], clr, cm[i], cmr)
         exact += f"{cl[i]:+.3e}" == f"{clr:+.3e}" and f"{cm[i]:+.3e}" == f"{cmr:+.3e}"
     print(f"[examples] {case}: {exact}/11 frequency lines identical as printed", flush=True)
+    if cp_file is not None:
+        # steady c_p and its transfer at k = 1, against the oracle's (tests/golden/
+        # make_cp_golden.py): in memory to 1e-10, and the exported file as printed
+        gold = np.load(GOLD / "response_Ma0p5_cp.npz")
+        cp = sd.compute_pressure_coefficients().cpu().numpy()
+        assert np.abs(cp - gold["cp"]).max() <= 1e-10 * np.abs(gold["cp"]).max()
+        _, got = _read_cp(cp_file)
+        assert got.shape == (4 * 128 * 128, 5)
+        ref = np.stack([np.repeat(gold["cp"].reshape(-1), 4),
+                        np.repeat(gold["dcp"].real.reshape(-1), 4),
+                        np.repeat(gold["dcp"].imag.reshape(-1), 4)], axis=1)
+        ulp = 10.0 ** (np.floor(np.log10(np.maximum(np.abs(ref), 1e-300))) - 5)
+        err = np.abs(got[:, 2:] - ref)
+        assert (err <= 1.01 * ulp / 2 + 1e-12).all(), float((err / ulp).max())
     if mach == 0.5:
         # batched GMRES with direct fallback and the sequential direct path agree
         from pyeulerale_amd.drivers import frequency_response
