@@ -111,6 +111,59 @@ PEA_HD void face_flux2(const pea_params& P, const T* LL, const T* L, const T* R,
     }
 }
 
+// face_flux2 with the stencil read through an accessor and the results handed to sinks:
+// st(slot, v) returns component v of slot 0..3 = ll, l, r, rr; outL(v, f) and outR(v, f) take
+// the two sides' fluxes.  Only the scalar coefficients and one component of the stencil are
+// live at a time (the states sit in LDS), which keeps the tangent types within registers.
+// Same expressions, same results as face_flux2.
+template <class T, class G, class St, class OutL, class OutR>
+PEA_HD void face_flux2_lds(const pea_params& P, const St& st, const T& pLL, const T& pL,
+                           const T& pR, const T& pRR, const T& cL, const T& cR, const G* n,
+                           const G* nR, const G* gv, bool dissL, bool dissR, OutL outL,
+                           OutR outR) {
+    T qL, qR;
+    {
+        const T L0 = st(1, 0), L1 = st(1, 1), L2 = st(1, 2);
+        qL = (n[0] * L1 + n[1] * L2) / L0;
+        const T R0 = st(2, 0), R1 = st(2, 1), R2 = st(2, 2);
+        qR = (n[0] * R1 + n[1] * R2) / R0;
+    }
+    const G vn = gv[0] * n[0] + gv[1] * n[1];
+    T e2L = lit<T>(0.0), e4L = lit<T>(0.0), e2R = lit<T>(0.0), e4R = lit<T>(0.0);
+    if (dissL || dissR) {
+        const G sn = ssqrt(n[0] * n[0] + n[1] * n[1]);
+        if (dissL) {
+            const T lamL = sabs(qL) + cL * sn;
+            const T lamR = sabs(qR) + cR * sn;
+            side_coefs(P, lamL, lamR, sensor(pLL, pL, pR), sensor(pL, pR, pRR), e2L, e4L);
+        }
+        if (dissR) {
+            // R's view: q' = (nR . j) / rho = -q up to the sign of an exact zero
+            const double sL = -sign1(val(nR[0]) * val(st(1, 1)) + val(nR[1]) * val(st(1, 2)));
+            const double sR = -sign1(val(nR[0]) * val(st(2, 1)) + val(nR[1]) * val(st(2, 2)));
+            const T lamL = abs_signed(qL, sL) + cL * sn;
+            const T lamR = abs_signed(qR, sR) + cR * sn;
+            side_coefs(P, lamR, lamL, sensor(pRR, pR, pL), sensor(pR, pL, pLL), e2R, e4R);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const T Lv = st(1, v), Rv = st(2, v);
+        const T phL = v == 0 ? Lv * qL : v == 3 ? (Lv + pL) * qL : Lv * qL + n[v - 1] * pL;
+        const T phR = v == 0 ? Rv * qR : v == 3 ? (Rv + pR) * qR : Rv * qR + n[v - 1] * pR;
+        const T cen = (phL + phR) / 2.0;
+        const T ale = (Lv + Rv) / 2.0 * vn;
+        T a = cen, b = -cen;
+        if (dissL || dissR) {
+            const T LLv = st(0, v), RRv = st(3, v);
+            if (dissL) a = a - (e2L * (Rv - Lv) - e4L * (RRv - 3.0 * Rv + 3.0 * Lv - LLv));
+            if (dissR) b = b - (e2R * (Lv - Rv) - e4R * (LLv - 3.0 * Lv + 3.0 * Rv - RRv));
+        }
+        outL(v, a - ale);
+        outR(v, b + ale);
+    }
+}
+
 // ---------------------------------------------------------------------------- reverse
 // Reverse of side_dissipation with seed db on D: accumulates into the adjoints of the side's
 // ll, l, r, rr states (sb*), of its two wave integrals (lamb_l, lamb_r) and of the four

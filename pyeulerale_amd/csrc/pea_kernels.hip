@@ -422,46 +422,49 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const Ld& ld,
 #pragma unroll
         for (int v = 0; v < 4; ++v) u[v] = lds_get<T>(sU + v * W * BOX, BOX, b);
     };
+    // stencil accessor over four box cells (slots ll, l, r, rr)
+    auto stencil = [&](int bLL, int bL, int bR, int bRR) {
+        const double* u = sU;
+        return [u, bLL, bL, bR, bRR](int slot, int v) -> T {
+            const int b = slot == 0 ? bLL : slot == 1 ? bL : slot == 2 ? bR : bRR;
+            return lds_get<T>(u + v * W * BOX, BOX, b);
+        };
+    };
     T S[4];   // running sum of this cell's four face fluxes
     const int bn = tn + 2;
     if (live) {
-        T uc[4];
-        box(m, bn, uc);
+        const int bc = bidx(m, bn);
         // ---- angular face n | n+1 (this cell's f face)
         {
-            T Rs[4], LLs[4], RRs[4], fR[4];
-            box(m, bn + 1, Rs);
-            box(m, bn - 1, LLs);
-            box(m, bn + 2, RRs);
             G nr[2], nr2[2], gv[2];
             edge(m, n + 1, m + 1, n + 1, nr, gv);     // fi -> fo
             nrm(m + 1, n + 1, m, n + 1, nr2);         // R's bo -> bi
-            face_flux2(P, LLs, uc, Rs, RRs, sP[bidx(m, bn - 1)], sP[bidx(m, bn)],
-                       sP[bidx(m, bn + 1)], sP[bidx(m, bn + 2)], sC[bidx(m, bn)],
-                       sC[bidx(m, bn + 1)], nr, nr2, gv, true, true, S, fR);
-            if (tn + 1 < TN && n + 1 < g.N)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) lds_put(sFb + v * W * NT, NT, tid + TM, fR[v]);
+            const int bLL = bidx(m, bn - 1), bR = bidx(m, bn + 1), bRR = bidx(m, bn + 2);
+            const bool pass = tn + 1 < TN && n + 1 < g.N;
+            face_flux2_lds(P, stencil(bLL, bc, bR, bRR), sP[bLL], sP[bc], sP[bR], sP[bRR],
+                           sC[bc], sC[bR], nr, nr2, gv, true, true,
+                           [&](int v, const T& f) { S[v] = f; },
+                           [&](int v, const T& f) {
+                               if (pass) lds_put(sFb + v * W * NT, NT, tid + TM, f);
+                           });
         }
         // ---- radial face m | m+1 (this cell's o face), or the far-field face
         if (m < g.M - 1) {
-            T Rs[4], LLs[4], RRs[4], fR[4], Fo[4];
-            box(m + 1, bn, Rs);
-            box(m - 1, bn, LLs);    // the wall ghost for m = 0
-            box(m + 2, bn, RRs);    // the far-field ghost for m + 1 = M - 1
             G nr[2], nr2[2], gv[2];
             edge(m + 1, n + 1, m + 1, n, nr, gv);     // fo -> bo
             nrm(m + 1, n, m + 1, n + 1, nr2);         // R's bi -> fi
-            face_flux2(P, LLs, uc, Rs, RRs, sP[bidx(m - 1, bn)], sP[bidx(m, bn)],
-                       sP[bidx(m + 1, bn)], sP[bidx(m + 2, bn)], sC[bidx(m, bn)],
-                       sC[bidx(m + 1, bn)], nr, nr2, gv, m != g.M - 2, m + 1 != 1, Fo, fR);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) S[v] = S[v] + Fo[v];
-            if (tm + 1 < TM)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) lds_put(sFi + v * W * NT, NT, tid + 1, fR[v]);
+            // m - 1 is the wall ghost row for m = 0, m + 2 the far-field row for m + 1 = M - 1
+            const int bLL = bidx(m - 1, bn), bR = bidx(m + 1, bn), bRR = bidx(m + 2, bn);
+            const bool pass = tm + 1 < TM;
+            face_flux2_lds(P, stencil(bLL, bc, bR, bRR), sP[bLL], sP[bc], sP[bR], sP[bRR],
+                           sC[bc], sC[bR], nr, nr2, gv, m != g.M - 2, m + 1 != 1,
+                           [&](int v, const T& f) { S[v] = S[v] + f; },
+                           [&](int v, const T& f) {
+                               if (pass) lds_put(sFi + v * W * NT, NT, tid + 1, f);
+                           });
         } else {
-            T gh[4], Fo[4];
+            T uc[4], gh[4], Fo[4];
+            box(m, bn, uc);
             box(g.M, bn, gh);
             G nr[2], gv[2];
             edge(m + 1, n + 1, m + 1, n, nr, gv);
@@ -471,22 +474,18 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const Ld& ld,
         }
         // ---- faces shared with the neighbouring tiles
         if (tn == 0) {   // angular face n-1 | n; normal L -> R is minus this cell's b normal
-            T Ls[4], LLs[4], RRs[4], fL[4], Fb[4];
-            box(m, bn - 1, Ls);
-            box(m, bn - 2, LLs);
-            box(m, bn + 1, RRs);
             G nr[2], nr2[2], gv[2];
             edge(m, n, m + 1, n, nr, gv);             // bi -> bo
             nrm(m + 1, n, m, n, nr2);                 // this cell's bo -> bi
-            face_flux2(P, LLs, Ls, uc, RRs, sP[bidx(m, bn - 2)], sP[bidx(m, bn - 1)],
-                       sP[bidx(m, bn)], sP[bidx(m, bn + 1)], sC[bidx(m, bn - 1)],
-                       sC[bidx(m, bn)], nr, nr2, gv, false, true, fL, Fb);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) S[v] = S[v] + Fb[v];
+            const int bLL = bidx(m, bn - 2), bL = bidx(m, bn - 1), bRR = bidx(m, bn + 1);
+            face_flux2_lds(P, stencil(bLL, bL, bc, bRR), sP[bLL], sP[bL], sP[bc], sP[bRR],
+                           sC[bL], sC[bc], nr, nr2, gv, false, true, [](int, const T&) {},
+                           [&](int v, const T& f) { S[v] = S[v] + f; });
         }
         if (tm == 0) {
             if (m == 0) {   // wall face: central against the wall ghost
-                T gh[4], Fi[4];
+                T uc[4], gh[4], Fi[4];
+                box(m, bn, uc);
                 box(-1, bn, gh);
                 G nr[2], gv[2];
                 edge(0, n, 0, n + 1, nr, gv);          // bi -> fi
@@ -494,18 +493,15 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const Ld& ld,
 #pragma unroll
                 for (int v = 0; v < 4; ++v) S[v] = S[v] + Fi[v];
             } else {        // radial face m-1 | m; normal L -> R is minus this cell's i normal
-                T Ls[4], LLs[4], RRs[4], fL[4], Fi[4];
-                box(m - 1, bn, Ls);
-                box(m - 2, bn, LLs);    // the wall ghost for m = 1
-                box(m + 1, bn, RRs);    // the far-field ghost for m = M - 1
                 G nr[2], nr2[2], gv[2];
                 edge(m, n + 1, m, n, nr, gv);          // fo -> bo of L
                 nrm(m, n, m, n + 1, nr2);              // this cell's bi -> fi
-                face_flux2(P, LLs, Ls, uc, RRs, sP[bidx(m - 2, bn)], sP[bidx(m - 1, bn)],
-                           sP[bidx(m, bn)], sP[bidx(m + 1, bn)], sC[bidx(m - 1, bn)],
-                           sC[bidx(m, bn)], nr, nr2, gv, false, m != 1, fL, Fi);
-#pragma unroll
-                for (int v = 0; v < 4; ++v) S[v] = S[v] + Fi[v];
+                // m - 2 is the wall ghost row for m = 1, m + 1 the far-field row for m = M - 1
+                const int bLL = bidx(m - 2, bn), bL = bidx(m - 1, bn), bRR = bidx(m + 1, bn);
+                face_flux2_lds(P, stencil(bLL, bL, bc, bRR), sP[bLL], sP[bL], sP[bc], sP[bRR],
+                               sC[bL], sC[bc], nr, nr2, gv, false, m != 1,
+                               [](int, const T&) {},
+                               [&](int v, const T& f) { S[v] = S[v] + f; });
             }
         }
     }
