@@ -414,7 +414,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": pmc.get("traffic_bytes_per_launch"),
-                     "kernel": "k_face_fwd<double, double, 32, 8, LdReal, StoreRealB>",
+                     "kernel": "k_face_fwd<double, double, 16, 16, LdReal, StoreRealB>",
                      "kernel_ms": kern_ms, "algorithmic_bytes_per_launch": bytes_launch,
                      "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/"
                                        "pmc_face_kernels_1025x513.md",

@@ -20,7 +20,7 @@ namespace pea {
 // (a, b, c) in this order.
 template <class T>
 PEA_HD T sensor(const T& pa, const T& pb, const T& pc) {
-    return sabs(pa - 2.0 * pb + pc) / (pa + 2.0 * pb + pc);
+    return qdiv(sabs(pa - 2.0 * pb + pc), pa + 2.0 * pb + pc);
 }
 
 // Dissipation of one side (f90:207-215): returns D with f -= D.
@@ -124,9 +124,9 @@ PEA_HD void face_flux2_lds(const pea_params& P, const St& st, const T& pLL, cons
     T qL, qR;
     {
         const T L0 = st(1, 0), L1 = st(1, 1), L2 = st(1, 2);
-        qL = (n[0] * L1 + n[1] * L2) / L0;
+        qL = qdiv(n[0] * L1 + n[1] * L2, L0);
         const T R0 = st(2, 0), R1 = st(2, 1), R2 = st(2, 2);
-        qR = (n[0] * R1 + n[1] * R2) / R0;
+        qR = qdiv(n[0] * R1 + n[1] * R2, R0);
     }
     const G vn = gv[0] * n[0] + gv[1] * n[1];
     T e2L = lit<T>(0.0), e4L = lit<T>(0.0), e2R = lit<T>(0.0), e4R = lit<T>(0.0);
@@ -179,7 +179,7 @@ PEA_HD void side_dissipation_rev(const pea_params& P, const T& lam_l, const T& l
     const T lam = take_l ? lam_l : lam_r;
     const T a1 = pll - 2.0 * pl + pr, b1 = pll + 2.0 * pl + pr;
     const T a2 = pl - 2.0 * pr + prr, b2 = pl + 2.0 * pr + prr;
-    const T s1 = sabs(a1) / b1, s2 = sabs(a2) / b2;
+    const T s1 = qdiv(sabs(a1), b1), s2 = qdiv(sabs(a2), b2);
     const bool take_s1 = maxb_takes_first(s1, s2);
     const T s = take_s1 ? s1 : s2;
     const T e2 = P.k2 * s * lam;
@@ -187,7 +187,7 @@ PEA_HD void side_dissipation_rev(const pea_params& P, const T& lam_l, const T& l
     const bool zero4 = maxb_takes_first(lit<T>(0.0), t4);
     const T e4 = zero4 ? lit<T>(0.0) : t4;
     const double sg1 = sign1(a1), sg2 = sign1(a2);
-    const T ib1 = 1.0 / b1, ib2 = 1.0 / b2;
+    const T ib1 = recip(b1), ib2 = recip(b2);
 #pragma unroll
     for (int k = 0; k < L; ++k) {
         T e2b = lit<T>(0.0), e4b = lit<T>(0.0);
@@ -250,9 +250,9 @@ PEA_HD void face_flux2_rev(const pea_params& P, const T* LL, const T* Lc, const 
                            const T& cL, const T& cR, const G* n, const G* nR, const G* gv,
                            bool dissL, bool dissR, const T (*fbL)[L], const T (*fbR)[L],
                            FaceAdj<T, L>& a) {
-    const T iL = 1.0 / Lc[0], iR = 1.0 / R[0];
-    const T qL = (n[0] * Lc[1] + n[1] * Lc[2]) / Lc[0];
-    const T qR = (n[0] * R[1] + n[1] * R[2]) / R[0];
+    const T iL = recip(Lc[0]), iR = recip(R[0]);
+    const T qL = qdiv(n[0] * Lc[1] + n[1] * Lc[2], Lc[0]);
+    const T qR = qdiv(n[0] * R[1] + n[1] * R[2], R[0]);
     const G vn = gv[0] * n[0] + gv[1] * n[1];
     T qLb[L], qRb[L];
 #pragma unroll
@@ -335,11 +335,17 @@ PEA_HD void face_flux2_rev(const pea_params& P, const T* LL, const T* Lc, const 
     }
 }
 
+// Pressure (gamma-1)(E - j.j/rho/2) of f90:128-133 with the face kernels' quotient.
+template <class T>
+PEA_HD T pressure_q(const T* u) {
+    return (kGamma - 1.0) * (u[3] - qdiv(u[1] * u[1] + u[2] * u[2], u[0]) / 2.0);
+}
+
 // Per-cell pressure and sound speed sqrt(gamma p / rho) (f90:128-133, 147).
 template <class T>
 PEA_HD void cell_pc(const T* u, T& p, T& c) {
-    p = pressure(u);
-    c = ssqrt(kGamma * p / u[0]);
+    p = pressure_q(u);
+    c = ssqrt(qdiv(kGamma * p, u[0]));
 }
 
 }  // namespace pea

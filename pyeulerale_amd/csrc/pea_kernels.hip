@@ -338,20 +338,38 @@ PEA_HD T lds_get(const double* base, int stride, int i) {
     return x;
 }
 
+// x of the lane `delta` below this one in the wave (this lane's own x for the lowest lanes);
+// every double of T moves separately.
+template <class T>
+__device__ __forceinline__ T shfl_up_t(const T& x, unsigned delta) {
+    constexpr int W = sizeof(T) / sizeof(double);
+    T r;
+    const double* p = reinterpret_cast<const double*>(&x);
+    double* q = reinterpret_cast<double*>(&r);
+#pragma unroll
+    for (int k = 0; k < W; ++k) q[k] = __shfl_up(p[k], delta, 64);
+    return r;
+}
+
 // One tile of TM x TN cells at tile coordinates (bx, by) of state bz; the body of the
 // single-grid and of the multi-grid kernels.
 template <class T, class G, int TM, int TN, class Ld, class Store>
 __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const Ld& ld,
                                               const Store& store, int bx, int by, int bz) {
     constexpr int NT = TM * TN, BM = TM + 4, BN = TN + 4, BOX = BM * BN;
+    static_assert(TM + TN <= NT, "the tile-edge faces take one thread each");
+    static_assert(64 % TM == 0 && NT % 64 == 0, "a wave holds whole tile columns");
+    constexpr int CPW = 64 / TM, SB = NT / 64 * TM;   // columns per wave; b hand-off slots
     // LDS holds, as structure of arrays of the doubles of T: the states of the tile grown by two
     // cells (with the wall and far-field ghosts in the grown rows m = -1 and m = M), and the
-    // fluxes handed to the neighbouring cell.  Every stencil read of the face evaluations is an
-    // LDS read; the global loads happen once, coalesced, in the staging loop.
+    // fluxes handed to a cell in another wave: the b faces of each wave's first column and the
+    // i faces of the tile's first row.  Every stencil read of the face evaluations is an LDS
+    // read; the global loads happen once, coalesced, in the staging loop.  Fluxes for a cell in
+    // the same wave move between lanes.
     constexpr int W = sizeof(T) / sizeof(double);
     __shared__ T sP[BOX], sC[BOX];
     __shared__ double sU[4 * W * BOX];
-    __shared__ double sFb[4 * W * NT], sFi[4 * W * NT];
+    __shared__ double sFb[4 * W * SB], sFi[4 * W * TN];
     const Grid g = ld.g;
     Ld l = ld;
     l.U = ld.U + (long)bz * 4 * g.C;
@@ -410,7 +428,7 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const Ld& ld,
             }
 #pragma unroll
             for (int v = 0; v < 4; ++v) lds_put(sU + v * W * BOX, BOX, i, gh[v]);
-            sP[i] = pressure(gh);
+            sP[i] = pressure_q(gh);
         }
     }
     __syncthreads();
@@ -430,38 +448,46 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const Ld& ld,
             return lds_get<T>(u + v * W * BOX, BOX, b);
         };
     };
-    T S[4];   // running sum of this cell's four face fluxes
+    T S[4];       // running sum of this cell's four face fluxes
+    T fb[4], fo[4];   // fluxes of this cell's f and o faces for the cells at n + 1 and m + 1
     const int bn = tn + 2;
+    const int cw = tn % CPW;   // column of this cell within its wave
+    // ---- angular face n | n+1 (this cell's f face)
     if (live) {
         const int bc = bidx(m, bn);
-        // ---- angular face n | n+1 (this cell's f face)
-        {
-            G nr[2], nr2[2], gv[2];
-            edge(m, n + 1, m + 1, n + 1, nr, gv);     // fi -> fo
-            nrm(m + 1, n + 1, m, n + 1, nr2);         // R's bo -> bi
-            const int bLL = bidx(m, bn - 1), bR = bidx(m, bn + 1), bRR = bidx(m, bn + 2);
-            const bool pass = tn + 1 < TN && n + 1 < g.N;
-            face_flux2_lds(P, stencil(bLL, bc, bR, bRR), sP[bLL], sP[bc], sP[bR], sP[bRR],
-                           sC[bc], sC[bR], nr, nr2, gv, true, true,
-                           [&](int v, const T& f) { S[v] = f; },
-                           [&](int v, const T& f) {
-                               if (pass) lds_put(sFb + v * W * NT, NT, tid + TM, f);
-                           });
-        }
-        // ---- radial face m | m+1 (this cell's o face), or the far-field face
+        G nr[2], nr2[2], gv[2];
+        edge(m, n + 1, m + 1, n + 1, nr, gv);     // fi -> fo
+        nrm(m + 1, n + 1, m, n + 1, nr2);         // R's bo -> bi
+        const int bLL = bidx(m, bn - 1), bR = bidx(m, bn + 1), bRR = bidx(m, bn + 2);
+        face_flux2_lds(P, stencil(bLL, bc, bR, bRR), sP[bLL], sP[bc], sP[bR], sP[bRR],
+                       sC[bc], sC[bR], nr, nr2, gv, true, true,
+                       [&](int v, const T& f) { S[v] = f; },
+                       [&](int v, const T& f) { fb[v] = f; });
+        // the cell at n + 1 starts the next wave: hand over through LDS
+        if (cw == CPW - 1 && tn + 1 < TN && n + 1 < g.N)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                lds_put(sFb + v * W * SB, SB, (tn + 1) / CPW * TM + tm, fb[v]);
+    }
+    // within the wave the cell at n + 1 is TM lanes up
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const T f = shfl_up_t(fb[v], TM);
+        if (live && cw > 0) S[v] = S[v] + f;
+    }
+    // ---- radial face m | m+1 (this cell's o face), or the far-field face
+    if (live) {
+        const int bc = bidx(m, bn);
         if (m < g.M - 1) {
             G nr[2], nr2[2], gv[2];
             edge(m + 1, n + 1, m + 1, n, nr, gv);     // fo -> bo
             nrm(m + 1, n, m + 1, n + 1, nr2);         // R's bi -> fi
             // m - 1 is the wall ghost row for m = 0, m + 2 the far-field row for m + 1 = M - 1
             const int bLL = bidx(m - 1, bn), bR = bidx(m + 1, bn), bRR = bidx(m + 2, bn);
-            const bool pass = tm + 1 < TM;
             face_flux2_lds(P, stencil(bLL, bc, bR, bRR), sP[bLL], sP[bc], sP[bR], sP[bRR],
                            sC[bc], sC[bR], nr, nr2, gv, m != g.M - 2, m + 1 != 1,
                            [&](int v, const T& f) { S[v] = S[v] + f; },
-                           [&](int v, const T& f) {
-                               if (pass) lds_put(sFi + v * W * NT, NT, tid + 1, f);
-                           });
+                           [&](int v, const T& f) { fo[v] = f; });
         } else {
             T uc[4], gh[4], Fo[4];
             box(m, bn, uc);
@@ -472,47 +498,66 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const Ld& ld,
 #pragma unroll
             for (int v = 0; v < 4; ++v) S[v] = S[v] + Fo[v];
         }
-        // ---- faces shared with the neighbouring tiles
-        if (tn == 0) {   // angular face n-1 | n; normal L -> R is minus this cell's b normal
+    }
+    // the cell at m + 1 is the next lane
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const T f = shfl_up_t(fo[v], 1);
+        if (live && tm > 0) S[v] = S[v] + f;
+    }
+    // ---- faces shared with the neighbouring tiles: the angular faces n0-1 | n0 of the tile's
+    // rows (threads 0..TM-1) and the radial faces m0-1 | m0 of its columns (threads TM..TM+TN-1).
+    // They take the first wave(s) only: a branch taken by one or two lanes of every wave would
+    // cost every wave a full face evaluation.  Their results reach the cells through LDS.
+    if (tid < TM) {   // angular face n0-1 | n0; normal L -> R is minus cell (mm, n0)'s b normal
+        const int mm = m0 + tid, nn = n0;
+        if (mm < g.M) {
             G nr[2], nr2[2], gv[2];
-            edge(m, n, m + 1, n, nr, gv);             // bi -> bo
-            nrm(m + 1, n, m, n, nr2);                 // this cell's bo -> bi
-            const int bLL = bidx(m, bn - 2), bL = bidx(m, bn - 1), bRR = bidx(m, bn + 1);
-            face_flux2_lds(P, stencil(bLL, bL, bc, bRR), sP[bLL], sP[bL], sP[bc], sP[bRR],
-                           sC[bL], sC[bc], nr, nr2, gv, false, true, [](int, const T&) {},
-                           [&](int v, const T& f) { S[v] = S[v] + f; });
+            edge(mm, nn, mm + 1, nn, nr, gv);           // bi -> bo
+            nrm(mm + 1, nn, mm, nn, nr2);               // the cell's bo -> bi
+            const int bLL = bidx(mm, 0), bL = bidx(mm, 1), bR = bidx(mm, 2), bRR = bidx(mm, 3);
+            face_flux2_lds(P, stencil(bLL, bL, bR, bRR), sP[bLL], sP[bL], sP[bR], sP[bRR],
+                           sC[bL], sC[bR], nr, nr2, gv, false, true, [](int, const T&) {},
+                           [&](int v, const T& f) { lds_put(sFb + v * W * SB, SB, tid, f); });
         }
-        if (tm == 0) {
-            if (m == 0) {   // wall face: central against the wall ghost
+    } else if (tid < TM + TN) {
+        const int j = tid - TM, mm = m0, nn = n0 + j, bnn = j + 2;
+        if (nn < g.N) {
+            if (mm == 0) {   // wall face: central against the wall ghost
                 T uc[4], gh[4], Fi[4];
-                box(m, bn, uc);
-                box(-1, bn, gh);
+                box(mm, bnn, uc);
+                box(-1, bnn, gh);
                 G nr[2], gv[2];
-                edge(0, n, 0, n + 1, nr, gv);          // bi -> fi
+                edge(0, nn, 0, nn + 1, nr, gv);          // bi -> fi
                 face_flux(P, uc, uc, gh, gh, nr, gv, false, Fi);
 #pragma unroll
-                for (int v = 0; v < 4; ++v) S[v] = S[v] + Fi[v];
-            } else {        // radial face m-1 | m; normal L -> R is minus this cell's i normal
+                for (int v = 0; v < 4; ++v) lds_put(sFi + v * W * TN, TN, j, Fi[v]);
+            } else {         // radial face m0-1 | m0; normal L -> R is minus the cell's i normal
                 G nr[2], nr2[2], gv[2];
-                edge(m, n + 1, m, n, nr, gv);          // fo -> bo of L
-                nrm(m, n, m, n + 1, nr2);              // this cell's bi -> fi
-                // m - 2 is the wall ghost row for m = 1, m + 1 the far-field row for m = M - 1
-                const int bLL = bidx(m - 2, bn), bL = bidx(m - 1, bn), bRR = bidx(m + 1, bn);
-                face_flux2_lds(P, stencil(bLL, bL, bc, bRR), sP[bLL], sP[bL], sP[bc], sP[bRR],
-                               sC[bL], sC[bc], nr, nr2, gv, false, m != 1,
+                edge(mm, nn + 1, mm, nn, nr, gv);        // fo -> bo of L
+                nrm(mm, nn, mm, nn + 1, nr2);            // the cell's bi -> fi
+                // mm - 2 is the wall ghost row for mm = 1, mm + 1 the far-field row for
+                // mm = M - 1
+                const int bLL = bidx(mm - 2, bnn), bL = bidx(mm - 1, bnn), bR = bidx(mm, bnn),
+                          bRR = bidx(mm + 1, bnn);
+                face_flux2_lds(P, stencil(bLL, bL, bR, bRR), sP[bLL], sP[bL], sP[bR], sP[bRR],
+                               sC[bL], sC[bR], nr, nr2, gv, false, mm != 1,
                                [](int, const T&) {},
-                               [&](int v, const T& f) { S[v] = S[v] + f; });
+                               [&](int v, const T& f) { lds_put(sFi + v * W * TN, TN, j, f); });
             }
         }
     }
     __syncthreads();
     if (live) {
-        if (tn > 0)
+        // faces handed over through LDS: the b face of a wave's first column, the i face of
+        // the tile's first row
+        if (cw == 0)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) S[v] = S[v] + lds_get<T>(sFb + v * W * NT, NT, tid);
-        if (tm > 0)
+            for (int v = 0; v < 4; ++v)
+                S[v] = S[v] + lds_get<T>(sFb + v * W * SB, SB, tn / CPW * TM + tm);
+        if (tm == 0)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) S[v] = S[v] + lds_get<T>(sFi + v * W * NT, NT, tid);
+            for (int v = 0; v < 4; ++v) S[v] = S[v] + lds_get<T>(sFi + v * W * TN, TN, tn);
         G x[4][2];
         vtx(m, n, x[BI]);
         vtx(m, n + 1, x[FI]);
@@ -520,10 +565,10 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const Ld& ld,
         vtx(m + 1, n, x[BO]);
         const G d1x = x[FO][0] - x[BI][0], d1z = x[FO][1] - x[BI][1];
         const G d2x = x[BO][0] - x[FI][0], d2z = x[BO][1] - x[FI][1];
-        const G area = sabs(d1x * d2z - d1z * d2x) / 2.0;
+        const G ia = recip(sabs(d1x * d2z - d1z * d2x) / 2.0);
         T R[4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) R[v] = -(S[v] / area);
+        for (int v = 0; v < 4; ++v) R[v] = -(S[v] * ia);
         store.put(bz, n * g.M + m, R);
     }
 }
@@ -1266,7 +1311,7 @@ k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __
             vtx(cm + 1, cn, x[BO]);
             const T d1x = x[FO][0] - x[BI][0], d1z = x[FO][1] - x[BI][1];
             const T d2x = x[BO][0] - x[FI][0], d2z = x[BO][1] - x[FI][1];
-            const T ia = 1.0 / (sabs(d1x * d2z - d1z * d2x) / 2.0);
+            const T ia = rcp_fast(sabs(d1x * d2z - d1z * d2x) / 2.0);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 sFB[v][0][i] = -(wseed[2 * ((long)v * g.C + c)] * ia);
@@ -1288,7 +1333,7 @@ k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __
             }
 #pragma unroll
             for (int v = 0; v < 4; ++v) sU[v][i] = gh[v];
-            sP[i] = pressure(gh);
+            sP[i] = pressure_q(gh);
         }
     }
     for (int i = tid; i < NC * 4 * L; i += NT) (&accU[0][0][0])[i] = 0.0;
@@ -1460,7 +1505,7 @@ k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __
                 pb[k] += a.p[1][k];
                 ghp[k] = gP_[side][tn][k] + a.p[2][k];
             }
-            pressure_rev<T, L>(gh, 1.0 / gh[0], ghp, ghb);
+            pressure_rev<T, L>(gh, rcp_fast(gh[0]), ghp, ghb);
             if (side == 0) {
                 T nb_[2][L], gvb_[2][L];
                 wall_ghost_rev<T, L>(u, nr, gv, ghb, ub, nb_, gvb_);
@@ -1472,11 +1517,11 @@ k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __
             }
         }
         // c = sqrt(gamma p / rho)
-        const T p = sP[b], cc = sC[b], iu = 1.0 / u[0];
-        const T a_ = kGamma * p / u[0];
+        const T p = sP[b], cc = sC[b], iu = rcp_fast(u[0]);
+        const T a_ = kGamma * p * iu, ic2 = rcp_fast(2.0 * cc);
 #pragma unroll
         for (int k = 0; k < L; ++k) {
-            const T ab = cb[k] / (2.0 * cc);
+            const T ab = cb[k] * ic2;
             pb[k] += kGamma * ab * iu;
             ub[0][k] -= ab * a_ * iu;
         }
@@ -1526,12 +1571,12 @@ int cell_block(long C) { return C >= 256L * 1024 ? 256 : 64; }
 // Tile shapes of the forward kernels (cells per workgroup, one thread per cell); the defaults
 // are the fastest measured on MI355X and may be overridden at build time.
 #ifndef PEA_FWD_TILE_M
-#define PEA_FWD_TILE_M 32
-#define PEA_FWD_TILE_N 8
+#define PEA_FWD_TILE_M 16
+#define PEA_FWD_TILE_N 16
 #endif
 #ifndef PEA_JVP_TILE_M
-#define PEA_JVP_TILE_M 32
-#define PEA_JVP_TILE_N 4
+#define PEA_JVP_TILE_M 16
+#define PEA_JVP_TILE_N 8
 #endif
 constexpr int kFwdTM = PEA_FWD_TILE_M, kFwdTN = PEA_FWD_TILE_N;
 constexpr int kJvpTM = PEA_JVP_TILE_M, kJvpTN = PEA_JVP_TILE_N;
@@ -1653,7 +1698,14 @@ int launch_vjp_shape(const pea_params& P, const Ld& ld, const double* w, double*
     return PEA_OK;
 }
 
-constexpr int kVjpFaceTM = 32, kVjpFaceTN = 5;
+// Tile of the face-based adjoint (cells per workgroup of 256 threads): its TM (TN + 3) angular
+// and (TM + 3) TN radial faces should each fit one round of 256 threads.  Overridable at build
+// time like the forward tiles.
+#ifndef PEA_VJP_TILE_M
+#define PEA_VJP_TILE_M 14
+#define PEA_VJP_TILE_N 15
+#endif
+constexpr int kVjpFaceTM = PEA_VJP_TILE_M, kVjpFaceTN = PEA_VJP_TILE_N;
 
 template <class T, class Ld>
 int launch_vjp(const pea_params& P, const Ld& ld, const double* w, double* gU, double* gX,

@@ -74,6 +74,22 @@ PEA_HD Cx ssqrt(Cx z) {
     return {fabs(z.i) / (2.0 * t), copysign(t, z.i)};
 }
 
+// Reciprocal for derivative factors: the hardware estimate refined by two Newton steps
+// (r <- r + r(1 - x r)), within a few ulp of 1/x for normal x.  Primal values keep IEEE division
+// so that every branch decision matches the residual; tangent and adjoint factors only need
+// relative accuracy far below the 1e-10 product tolerance.
+PEA_HD double rcp_fast(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+#else
+    return 1.0 / x;
+#endif
+}
+
 // ------------------------------------------------------------------------------- dual
 template <int K>
 struct Dual {
@@ -136,15 +152,17 @@ template <int K>
 PEA_HD Dual<K> operator/(const Dual<K>& a, double s) {
     Dual<K> r;
     r.v = a.v / s;
+    // a literal divisor (the halves of the central and ALE terms) folds to an exact constant
+    const double inv = __builtin_constant_p(s) ? 1.0 / s : rcp_fast(s);
 #pragma unroll
-    for (int k = 0; k < K; ++k) r.d[k] = a.d[k] / s;
+    for (int k = 0; k < K; ++k) r.d[k] = a.d[k] * inv;
     return r;
 }
 template <int K>
 PEA_HD Dual<K> operator/(const Dual<K>& a, const Dual<K>& b) {
     Dual<K> r;
     r.v = a.v / b.v;
-    const double inv = 1.0 / b.v;
+    const double inv = rcp_fast(b.v);
 #pragma unroll
     for (int k = 0; k < K; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) * inv;
     return r;
@@ -171,7 +189,7 @@ template <int K>
 PEA_HD Dual<K> operator/(double s, const Dual<K>& b) {
     Dual<K> r;
     r.v = s / b.v;
-    const double g = -r.v / b.v;
+    const double g = -r.v * rcp_fast(b.v);
 #pragma unroll
     for (int k = 0; k < K; ++k) r.d[k] = g * b.d[k];
     return r;
@@ -197,7 +215,7 @@ template <int K>
 PEA_HD Dual<K> ssqrt(const Dual<K>& x) {
     Dual<K> r;
     r.v = sqrt(x.v);
-    const double h = 0.5 / r.v;
+    const double h = 0.5 * rcp_fast(r.v);
 #pragma unroll
     for (int k = 0; k < K; ++k) r.d[k] = x.d[k] * h;
     return r;
@@ -205,6 +223,37 @@ PEA_HD Dual<K> ssqrt(const Dual<K>& x) {
 template <int K>
 PEA_HD double sign1(const Dual<K>& x) {
     return copysign(1.0, x.v);
+}
+
+// a/b through rcp_fast, within about 1.5 ulp of the correctly rounded quotient: the face-based
+// kernels' divisions (the reference's expressions, one rounding apart from its IEEE divisions;
+// R agrees to a few ulp, far inside the 1e-12 parity tolerance).
+PEA_HD double qdiv(double a, double b) { return a * rcp_fast(b); }
+template <int K>
+PEA_HD Dual<K> qdiv(const Dual<K>& a, const Dual<K>& b) {
+    const double rb = rcp_fast(b.v);
+    Dual<K> r;
+    r.v = a.v * rb;
+#pragma unroll
+    for (int k = 0; k < K; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) * rb;
+    return r;
+}
+template <int K>
+PEA_HD Dual<K> qdiv(const Dual<K>& a, double b) {
+    const double rb = rcp_fast(b);
+    Dual<K> r;
+    r.v = a.v * rb;
+#pragma unroll
+    for (int k = 0; k < K; ++k) r.d[k] = a.d[k] * rb;
+    return r;
+}
+
+// 1/x where it only scales derivatives (adjoint factors): rcp_fast for double; for Dual<K>
+// the quotient's own rule.
+PEA_HD double recip(double x) { return rcp_fast(x); }
+template <int K>
+PEA_HD Dual<K> recip(const Dual<K>& x) {
+    return 1.0 / x;
 }
 
 // Constant of scalar type T.
