@@ -78,8 +78,9 @@ int pea_compute_odes_batch(const pea_params *prm, double mach, int M, int N, int
                            const double *X, const double *V, const double *U, double *R,
                            void *stream);
 
-/* Many grids of uneven size in one persistent launch (the reference evaluates one grid per
- * compute_odes call, f90:496-537).  Results are bitwise those of one single-grid call per grid.
+/* Many grids of uneven size in one launch, one workgroup per tile (the reference evaluates one
+ * grid per compute_odes call, f90:496-537).  Results are bitwise those of one single-grid call
+ * per grid.
  * pea_multi_prepare: uploads the launch's descriptor table built from the host array `grids`
  *   (G entries) into `workspace` (device, pea_multi_workspace_bytes(G) bytes) and returns the
  *   tile count; it synchronises the stream.  tangent = 0: residuals (R written); tangent = 1:

@@ -62,13 +62,20 @@ struct Grid {
 static Grid make_grid(int M, int N) { return Grid{M, N, M * N, (M + 1) * (N + 1)}; }
 
 PEA_HD int wrapn(int n, int N) { return n < 0 ? n + N : (n >= N ? n - N : n); }
+// n mod N: one compare-and-add for n within a period of [0, N) (stencil offsets of a few
+// cells), the integer remainder only for grids narrower than a tile's halo.
+PEA_HD int wrap_any(int n, int N) {
+    n = wrapn(n, N);
+    if ((unsigned)n >= (unsigned)N) n = (n % N + N) % N;
+    return n;
+}
 
 // MI355X dispatches workgroup b to XCD b mod 8, and each XCD has its own L2.  Tiles that share
 // halo cells should run on one XCD: xcd_tile() maps the linear workgroup id to a tile id so that
 // XCD x walks the contiguous tile range [start(x), start(x+1)) (a bijection for any total).
 constexpr int kXcdCount = 8;
-__device__ __forceinline__ long xcd_tile(long b, long total) {
-    const long q = total / kXcdCount, r = total % kXcdCount, x = b % kXcdCount;
+__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned total) {
+    const unsigned q = total / kXcdCount, r = total % kXcdCount, x = b % kXcdCount;
     return x * q + (x < r ? x : r) + b / kXcdCount;
 }
 
@@ -354,7 +361,7 @@ __device__ __forceinline__ T shfl_up_t(const T& x, unsigned delta) {
 // One tile of TM x TN cells at tile coordinates (bx, by) of state bz; the body of the
 // single-grid and of the multi-grid kernels.
 template <class T, class G, int TM, int TN, class Ld, class Store>
-__device__ __forceinline__ void face_fwd_tile(const pea_params& P, const Ld& ld,
+__device__ __forceinline__ void face_fwd_tile(const pea_params& P, const FsDir& fs, const Ld& ld,
                                               const Store& store, int bx, int by, int bz) {
     constexpr int NT = TM * TN, BM = TM + 4, BN = TN + 4, BOX = BM * BN;
     static_assert(TM + TN <= NT, "the tile-edge faces take one thread each");
@@ -405,7 +412,7 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const Ld& ld,
     for (int i = tid; i < BOX; i += NT) {
         const int bm = i % BM, bn = i / BM;
         const int cm = m0 - 2 + bm;
-        const int cn = ((n0 - 2 + bn) % g.N + g.N) % g.N;
+        const int cn = wrap_any(n0 - 2 + bn, g.N);
         if (cm >= 0 && cm < g.M) {
             T u[4];
             l.state(cn * g.M + cm, u);
@@ -422,7 +429,7 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const Ld& ld,
                 wall_ghost(u, nr, gv, gh);
             } else {
                 decltype(l.machv()) inf[4];
-                free_stream(P, mach, inf);
+                free_stream_dir(mach, fs.ca, fs.sa, inf);
 #pragma unroll
                 for (int v = 0; v < 4; ++v) gh[v] = 2.0 * inf[v] - u[v];
             }
@@ -575,10 +582,11 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const Ld& ld,
 
 template <class T, class G, int TM, int TN, class Ld, class Store>
 __global__ void __launch_bounds__(TM * TN)
-k_face_fwd(pea_params P, Ld ld, Store store) {
-    const long nx = gridDim.x, ny = gridDim.y;
-    const long t = xcd_tile(blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z), nx * ny * gridDim.z);
-    face_fwd_tile<T, G, TM, TN>(P, ld, store, (int)(t % nx), (int)(t / nx % ny),
+k_face_fwd(pea_params P, FsDir fs, Ld ld, Store store) {
+    const unsigned nx = gridDim.x, ny = gridDim.y;
+    const unsigned t = xcd_tile(blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z),
+                                nx * ny * gridDim.z);
+    face_fwd_tile<T, G, TM, TN>(P, fs, ld, store, (int)(t % nx), (int)(t / nx % ny),
                                 (int)(t / (nx * ny)));
 }
 
@@ -1234,7 +1242,8 @@ k_vjp(pea_params P, Ld ld, const double* __restrict__ wseed, double* __restrict_
 // Requires TN + 3 <= N (no face is reached twice through the periodic seam).
 template <int TM, int TN, int NT>
 __global__ void __launch_bounds__(NT)
-k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __restrict__ gU) {
+k_face_vjp(pea_params P, FsDir fs, LdReal ld, const double* __restrict__ wseed,
+           double* __restrict__ gU) {
     using T = double;
     constexpr int L = 2;
     constexpr int BM = TM + 6, BN = TN + 6, BOX = BM * BN, NC = TM * TN;
@@ -1247,8 +1256,8 @@ k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __
     __shared__ T accU[4][L][NC], accP[L][NC], accC[L][NC];
     __shared__ T gU_[2][TN][4][L], gP_[2][TN][L];   // ghost adjoints of the tile's columns
     const Grid g = ld.g;
-    const long tile = xcd_tile(blockIdx.x + (long)gridDim.x * blockIdx.y,
-                               (long)gridDim.x * gridDim.y);   // XCD-contiguous tiles
+    const unsigned tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y,
+                                   gridDim.x * gridDim.y);   // XCD-contiguous tiles
     const int m0 = (int)(tile % gridDim.x) * TM, n0 = (int)(tile / gridDim.x) * TN;
     const int tid = threadIdx.x;
 
@@ -1296,7 +1305,7 @@ k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __
     for (int i = tid; i < BOX; i += NT) {
         const int bm = i % BM, bn = i / BM;
         const int cm = m0 - 3 + bm;
-        const int cn = ((n0 - 3 + bn) % g.N + g.N) % g.N;
+        const int cn = wrap_any(n0 - 3 + bn, g.N);
         if (cm >= 0 && cm < g.M) {
             T u[4];
             const int c = cn * g.M + cm;
@@ -1327,7 +1336,7 @@ k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __
                 wall_ghost(u, nr, gv, gh);
             } else {
                 T inf[4];
-                free_stream(P, ld.mach, inf);
+                free_stream_dir(ld.mach, fs.ca, fs.sa, inf);
 #pragma unroll
                 for (int v = 0; v < 4; ++v) gh[v] = 2.0 * inf[v] - u[v];
             }
@@ -1363,7 +1372,7 @@ k_face_vjp(pea_params P, LdReal ld, const double* __restrict__ wseed, double* __
             const int tm = f % TM, j = f / TM;
             const int m = m0 + tm;
             const bool act = f < NF && m < g.M;
-            const int Lc = ((n0 - 2 + j) % g.N + g.N) % g.N;
+            const int Lc = wrap_any(n0 - 2 + j, g.N);
             FaceAdj<T, L> a;
             if (act) {
                 face_adj_zero(a);
@@ -1586,40 +1595,37 @@ constexpr int kJvpTM = PEA_JVP_TILE_M, kJvpTN = PEA_JVP_TILE_N;
 // plus the grid's first tile in the launch-wide tile numbering.
 struct GridDesc {
     pea_grid d;
+    FsDir fs;
     int tiles_m, tile0;
 };
 
-constexpr int kXcds = 8;   // MI355X: workgroup i runs on XCD i mod 8, each XCD with its own L2
 
-// Persistent launch over the tiles of G grids.  The tile range is cut into kXcds contiguous
-// segments; the workgroups of one XCD walk one segment, so neighbouring tiles of a grid (which
-// share their halos) are read through the same L2.  gridDim.x is a multiple of kXcds.
+// One workgroup per tile of G grids.  Tiles are numbered grid after grid; xcd_tile() gives each
+// XCD a contiguous range of them, so neighbouring tiles of a grid (which share their halos) are
+// read through the same L2.  gridDim.x is the total tile count; a workgroup finds its grid by
+// binary search over the first-tile table.  (A persistent loop over tiles would keep every
+// tile-invariant address and constant live across the loop.)
 template <bool TANGENT>
 __global__ void __launch_bounds__(TANGENT ? kJvpTM * kJvpTN : kFwdTM * kFwdTN)
-k_face_fwd_multi(const GridDesc* __restrict__ desc, int G, int total) {
-    const int xcd = blockIdx.x % kXcds, lane = blockIdx.x / kXcds, lanes = gridDim.x / kXcds;
-    const int seg = (total + kXcds - 1) / kXcds;
-    const int t_end = min(total, (xcd + 1) * seg);
-    for (int t = xcd * seg + lane; t < t_end; t += lanes) {
-        int lo = 0, hi = G - 1;   // last grid whose first tile is <= t
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (desc[mid].tile0 <= t) lo = mid;
-            else hi = mid - 1;
-        }
-        const GridDesc e = desc[lo];
-        const Grid g{e.d.M, e.d.N, e.d.M * e.d.N, (e.d.M + 1) * (e.d.N + 1)};
-        const int tl = t - e.tile0;
-        if (TANGENT)
-            face_fwd_tile<Dual<2>, double, kJvpTM, kJvpTN>(
-                e.d.prm, LdDualU<2>{e.d.X, e.d.V, e.d.U, e.d.dU, e.d.mach, g},
-                StoreDualB<2>{e.d.dR, e.d.R, g.C}, tl % e.tiles_m, tl / e.tiles_m, 0);
-        else
-            face_fwd_tile<double, double, kFwdTM, kFwdTN>(
-                e.d.prm, LdReal{e.d.X, e.d.V, e.d.U, e.d.mach, g}, StoreRealB{e.d.R, g.C},
-                tl % e.tiles_m, tl / e.tiles_m, 0);
-        __syncthreads();   // the next tile restages the LDS
+k_face_fwd_multi(const GridDesc* __restrict__ desc, int G) {
+    const int t = (int)xcd_tile(blockIdx.x, gridDim.x);
+    int lo = 0, hi = G - 1;   // last grid whose first tile is <= t
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (desc[mid].tile0 <= t) lo = mid;
+        else hi = mid - 1;
     }
+    const GridDesc e = desc[lo];
+    const Grid g{e.d.M, e.d.N, e.d.M * e.d.N, (e.d.M + 1) * (e.d.N + 1)};
+    const int tl = t - e.tile0;
+    if (TANGENT)
+        face_fwd_tile<Dual<2>, double, kJvpTM, kJvpTN>(
+            e.d.prm, e.fs, LdDualU<2>{e.d.X, e.d.V, e.d.U, e.d.dU, e.d.mach, g},
+            StoreDualB<2>{e.d.dR, e.d.R, g.C}, tl % e.tiles_m, tl / e.tiles_m, 0);
+    else
+        face_fwd_tile<double, double, kFwdTM, kFwdTN>(
+            e.d.prm, e.fs, LdReal{e.d.X, e.d.V, e.d.U, e.d.mach, g}, StoreRealB{e.d.R, g.C},
+            tl % e.tiles_m, tl / e.tiles_m, 0);
 }
 dim3 fwd_grid(const Grid& g, int nstates, int tm = kFwdTM, int tn = kFwdTN) {
     return dim3((g.M + tm - 1) / tm, (g.N + tn - 1) / tn, nstates);
@@ -1714,7 +1720,7 @@ int launch_vjp(const pea_params& P, const Ld& ld, const double* w, double* gU, d
         if (gU && !gX && !gV && !gmach && ld.g.N >= kVjpFaceTN + 3 && g_face_vjp) {
             const dim3 grid((ld.g.M + kVjpFaceTM - 1) / kVjpFaceTM,
                             (ld.g.N + kVjpFaceTN - 1) / kVjpFaceTN);
-            k_face_vjp<kVjpFaceTM, kVjpFaceTN, 256><<<grid, 256, 0, st>>>(P, ld, w, gU);
+            k_face_vjp<kVjpFaceTM, kVjpFaceTN, 256><<<grid, 256, 0, st>>>(P, fs_dir(P), ld, w, gU);
             PEA_LAUNCH_CHECK("k_face_vjp");
             return PEA_OK;
         }
@@ -1803,7 +1809,7 @@ int pea_compute_odes(const pea_params* prm, double mach, int M, int N, const dou
     const Grid g = make_grid(M, N);
     k_face_fwd<double, double, kFwdTM, kFwdTN><<<fwd_grid(g, 1), kFwdTM * kFwdTN, 0,
                                                  as_stream(stream)>>>(
-        *prm, LdReal{X, V, U, mach, g}, StoreRealB{R, g.C});
+        *prm, fs_dir(*prm), LdReal{X, V, U, mach, g}, StoreRealB{R, g.C});
     PEA_LAUNCH_CHECK("k_face_fwd");
     return PEA_OK;
 }
@@ -1817,7 +1823,7 @@ int pea_compute_odes_batch(const pea_params* prm, double mach, int M, int N, int
     const Grid g = make_grid(M, N);
     k_face_fwd<double, double, kFwdTM, kFwdTN><<<fwd_grid(g, nstates), kFwdTM * kFwdTN, 0,
                                                  as_stream(stream)>>>(
-        *prm, LdReal{X, V, U, mach, g}, StoreRealB{R, g.C});
+        *prm, fs_dir(*prm), LdReal{X, V, U, mach, g}, StoreRealB{R, g.C});
     PEA_LAUNCH_CHECK("k_face_fwd");
     return PEA_OK;
 }
@@ -1837,6 +1843,7 @@ int pea_multi_prepare(int G, const pea_grid* grids, int tangent, void* workspace
         if (!q.X || !q.V || !q.U || (tangent ? (!q.dU || !q.dR) : !q.R))
             return fail(PEA_ERR_NULL, "null array in a grid descriptor");
         d[i].d = q;
+        d[i].fs = fs_dir(q.prm);
         d[i].tiles_m = (q.M + tm - 1) / tm;
         d[i].tile0 = (int)total;
         total += (long long)d[i].tiles_m * ((q.N + tn - 1) / tn);
@@ -1856,27 +1863,11 @@ int pea_multi_launch(int G, int total_tiles, int tangent, const void* workspace,
     if (G < 1 || total_tiles < 1) return fail(PEA_ERR_DIMS, "no tiles");
     if (!workspace) return fail(PEA_ERR_NULL, "null argument");
     const int nt = tangent ? kJvpTM * kJvpTN : kFwdTM * kFwdTN;
-    static int cus = 0, per_cu[2] = {0, 0};
-    if (!cus) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-            return fail(PEA_ERR_HIP, "device query");
-    }
-    int& pc = per_cu[tangent ? 1 : 0];
-    if (!pc) {
-        hipError_t oc = tangent
-            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, k_face_fwd_multi<true>, nt, 0)
-            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, k_face_fwd_multi<false>, nt, 0);
-        if (oc != hipSuccess || pc < 1) pc = 1;
-    }
-    long long nb = std::min<long long>(total_tiles, (long long)cus * pc);
-    nb = std::max<long long>(kXcds, (nb + kXcds - 1) / kXcds * kXcds);
     const GridDesc* d = static_cast<const GridDesc*>(workspace);
     if (tangent)
-        k_face_fwd_multi<true><<<(unsigned)nb, nt, 0, as_stream(stream)>>>(d, G, total_tiles);
+        k_face_fwd_multi<true><<<(unsigned)total_tiles, nt, 0, as_stream(stream)>>>(d, G);
     else
-        k_face_fwd_multi<false><<<(unsigned)nb, nt, 0, as_stream(stream)>>>(d, G, total_tiles);
+        k_face_fwd_multi<false><<<(unsigned)total_tiles, nt, 0, as_stream(stream)>>>(d, G);
     PEA_LAUNCH_CHECK("k_face_fwd_multi");
     return PEA_OK;
 }
@@ -1913,11 +1904,12 @@ int pea_jvp_odes(const pea_params* prm, double mach, int M, int N, const double*
     if (dU && !dX && !dV && !dmach)
         k_face_fwd<Dual<2>, double, kJvpTM, kJvpTN><<<fwd_grid(g, 1, kJvpTM, kJvpTN),
                                                       kJvpTM * kJvpTN, 0, as_stream(stream)>>>(
-            *prm, LdDualU<2>{X, V, U, dU, mach, g}, StoreDualB<2>{dR, R, g.C});
+            *prm, fs_dir(*prm), LdDualU<2>{X, V, U, dU, mach, g}, StoreDualB<2>{dR, R, g.C});
     else
         k_face_fwd<Dual<2>, Dual<2>, kJvpTM, kJvpTN><<<fwd_grid(g, 1, kJvpTM, kJvpTN),
                                                        kJvpTM * kJvpTN, 0, as_stream(stream)>>>(
-            *prm, LdDual<2>{X, V, U, dX, dV, dU, dmach, mach, g}, StoreDualB<2>{dR, R, g.C});
+            *prm, fs_dir(*prm), LdDual<2>{X, V, U, dX, dV, dU, dmach, mach, g},
+            StoreDualB<2>{dR, R, g.C});
     PEA_LAUNCH_CHECK("k_face_fwd");
     return PEA_OK;
 }

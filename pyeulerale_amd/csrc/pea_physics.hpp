@@ -78,15 +78,28 @@ PEA_HD void face_flux(const pea_params& P, const T* ll, const T* l, const T* r, 
     for (int v = 0; v < 4; ++v) f[v] = f[v] - (l[v] + r[v]) / 2.0 * vn;
 }
 
-// Free-stream state for Mach number `mach` (f90:250-259).
+// Direction (cos, sin) of the free stream for the angle of attack in degrees (f90:250-259).
+struct FsDir {
+    double ca, sa;
+};
+inline FsDir fs_dir(const pea_params& P) {
+    const double a = kDegToRad * P.aoa_deg;
+    return FsDir{cos(a), sin(a)};
+}
+
+// Free-stream state for Mach number `mach` along direction (ca, sa) (f90:250-259).
+template <class T>
+PEA_HD void free_stream_dir(const T& mach, double ca, double sa, T* u) {
+    const T spd = mach * sqrt(kGamma);
+    u[0] = lit<T>(1.0);
+    u[1] = spd * ca;
+    u[2] = spd * sa;
+    u[3] = spd * spd / 2.0 + 1.0 / (kGamma - 1.0);
+}
 template <class T>
 PEA_HD void free_stream(const pea_params& P, const T& mach, T* u) {
-    const T spd = mach * sqrt(kGamma);
     const double a = kDegToRad * P.aoa_deg;
-    u[0] = lit<T>(1.0);
-    u[1] = spd * cos(a);
-    u[2] = spd * sin(a);
-    u[3] = spd * spd / 2.0 + 1.0 / (kGamma - 1.0);
+    free_stream_dir(mach, cos(a), sin(a), u);
 }
 
 // Wall ghost state (f90:231-240).

@@ -179,7 +179,7 @@ class MultiGrid:
 
 
 def compute_odes_multi(items):
-    """Residuals of many grids of uneven size in one persistent launch.  ``items`` is a list of
+    """Residuals of many grids of uneven size in one launch.  ``items`` is a list of
     (params, mach, X, V, U) in storage layout; returns the list of R."""
     return MultiGrid(items).run()
 

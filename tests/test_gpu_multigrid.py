@@ -1,5 +1,5 @@
 """GPU tests of the multi-grid launch: many O-grids of uneven size (different thickness, size,
-Mach, angle of attack, JST constants and pitch motion) in one persistent launch give bitwise the
+Mach, angle of attack, JST constants and pitch motion) in one launch give bitwise the
 results of one single-grid call per grid, for the residual and for the states tangent."""
 
 from __future__ import annotations
