@@ -190,7 +190,7 @@ PEA_HD void side_dissipation_rev(const pea_params& P, const T& lam_l, const T& l
     const T ib1 = recip(b1), ib2 = recip(b2);
 #pragma unroll
     for (int k = 0; k < L; ++k) {
-        T e2b = lit<T>(0.0), e4b = lit<T>(0.0);
+        T e2b = adj_zero<T>(), e4b = adj_zero<T>();
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const T d = db[v][k];
@@ -235,9 +235,9 @@ PEA_HD void face_adj_zero(FaceAdj<T, L>& a) {
 #pragma unroll
         for (int k = 0; k < L; ++k) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) a.u[j][v][k] = lit<T>(0.0);
-            a.p[j][k] = lit<T>(0.0);
-            if (j < 2) a.c[j][k] = lit<T>(0.0);
+            for (int v = 0; v < 4; ++v) a.u[j][v][k] = adj_zero<T>();
+            a.p[j][k] = adj_zero<T>();
+            if (j < 2) a.c[j][k] = adj_zero<T>();
         }
 }
 
@@ -257,8 +257,8 @@ PEA_HD void face_flux2_rev(const pea_params& P, const T* LL, const T* Lc, const 
     T qLb[L], qRb[L];
 #pragma unroll
     for (int k = 0; k < L; ++k) {
-        qLb[k] = lit<T>(0.0);
-        qRb[k] = lit<T>(0.0);
+        qLb[k] = adj_zero<T>();
+        qRb[k] = adj_zero<T>();
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             // f_L = cen - D_L - ale,  f_R = -cen - D_R + ale
@@ -283,7 +283,7 @@ PEA_HD void face_flux2_rev(const pea_params& P, const T* LL, const T* Lc, const 
             T db[4][L], lbl[L], lbr[L];
 #pragma unroll
             for (int k = 0; k < L; ++k) {
-                lbl[k] = lbr[k] = lit<T>(0.0);
+                lbl[k] = lbr[k] = adj_zero<T>();
 #pragma unroll
                 for (int v = 0; v < 4; ++v) db[v][k] = -fbL[v][k];
             }
@@ -306,7 +306,7 @@ PEA_HD void face_flux2_rev(const pea_params& P, const T* LL, const T* Lc, const 
             T db[4][L], lbl[L], lbr[L];
 #pragma unroll
             for (int k = 0; k < L; ++k) {
-                lbl[k] = lbr[k] = lit<T>(0.0);
+                lbl[k] = lbr[k] = adj_zero<T>();
 #pragma unroll
                 for (int v = 0; v < 4; ++v) db[v][k] = -fbR[v][k];
             }

@@ -1388,7 +1388,7 @@ k_face_vjp(pea_params P, FsDir fs, LdReal ld, const double* __restrict__ wseed,
                 T nL[2], nRn[2], gv[2];
                 edge(m, Lc + 1, m + 1, Lc + 1, nL, gv);     // L's fi -> fo
                 nrm(m + 1, Rc, m, Rc, nRn);                 // R's bo -> bi
-                const T zero[4][L] = {};
+                const T zero[4][L] = {{-0.0, -0.0}, {-0.0, -0.0}, {-0.0, -0.0}, {-0.0, -0.0}};
                 if (Lc + 1 == g.N) {
                     // seam: L's face uses vertex column N, R's face column 0 -- reverse each
                     // side with its own geometry
@@ -1499,7 +1499,7 @@ k_face_vjp(pea_params P, FsDir fs, LdReal ld, const double* __restrict__ wseed,
             // central face (cell, ghost) seen from the cell: f_L of face_flux2 with L = cell
             FaceAdj<T, L> a;
             face_adj_zero(a);
-            const T zero[4][L] = {};
+            const T zero[4][L] = {{-0.0, -0.0}, {-0.0, -0.0}, {-0.0, -0.0}, {-0.0, -0.0}};
             face_flux2_rev<T, T, L>(P, u, u, gh, gh, sP[b], sP[b], sP[bidx(side ? g.M : -1, bn)],
                                     sP[bidx(side ? g.M : -1, bn)], sC[b], sC[b], nr, nr, gv,
                                     false, false, seed(b).w, zero, a);
@@ -1585,7 +1585,7 @@ int cell_block(long C) { return C >= 256L * 1024 ? 256 : 64; }
 #endif
 #ifndef PEA_JVP_TILE_M
 #define PEA_JVP_TILE_M 16
-#define PEA_JVP_TILE_N 8
+#define PEA_JVP_TILE_N 16
 #endif
 constexpr int kFwdTM = PEA_FWD_TILE_M, kFwdTN = PEA_FWD_TILE_N;
 constexpr int kJvpTM = PEA_JVP_TILE_M, kJvpTN = PEA_JVP_TILE_N;

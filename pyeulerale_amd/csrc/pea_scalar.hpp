@@ -274,6 +274,13 @@ PEA_HD T lit(double v) {
     return Lit<T>::make(v);
 }
 
+// Start value of an adjoint sum: -0.0, the identity of IEEE addition (-0 + x == x for every x,
+// unlike +0 + (-0)), so a sum that starts from it folds its first addition away.
+template <class T>
+PEA_HD T adj_zero() {
+    return lit<T>(-0.0);
+}
+
 // Biased maximum: x only if val(x) exceeds val(y) by more than 1e-8, else y (f90:111-120).
 // The bias makes the choice depend on argument order; callers keep the reference's order.
 template <class T>
