@@ -23,6 +23,10 @@ CASES = {
     "tiny": (9, 8, dict()),
     "small_pitch": (33, 17, dict(aoa=0.0, alpha=0.01, alpha_dot=0.004, thickness=0.04)),
     "c1": (129, 65, dict()),
+    # the face kernels at their limits: N = 18 angular cells (the adjoint tile's width + 3, so
+    # its grown tile wraps onto itself) and M = 8 < a tile's rows, so one tile holds the wall
+    # and far-field rows and the last tile column is partial and wraps the seam
+    "narrow": (19, 9, dict(alpha=0.02, alpha_dot=0.01)),
 }
 
 
@@ -90,7 +94,7 @@ def _linearization(orc, X, V, U, mach, aoa):
     )
 
 
-@pytest.mark.parametrize("name", ["tiny", "small_pitch", "c1"])
+@pytest.mark.parametrize("name", ["tiny", "small_pitch", "c1", "narrow"])
 def test_jvp_vjp_against_complex_step_blocks(name, orc, K):
     X, V, U, mach, aoa = setup_case(name)
     g, p, Xd, Vd, Ud = _dev(K, X, V, U, aoa)
