@@ -14,6 +14,10 @@ Also reported (same JSON line):
 * per-product times of the hand-written tangent (JVP) and adjoint (VJP) against the GPU
   complex-step comparators (directional complex step, and the reference's assembled-block
   products), with their errors, at 257x129 and 1025x513;
+* the products along every input (Ma, U, X, V; tangents and adjoints) on the 257x129 pitching
+  workload against the complex-step paths;
+* second-order gradients (tangent-over-adjoint) at 513x257 against the reference's method,
+  central differences of complex-step Jacobian products;
 * batched GMRES Arnoldi-step throughput for 32 shifts at 1025x513;
 * 64 O-grids of uneven size in one multi-grid launch against one launch per grid;
 * the response example's steady state and 32-frequency response on the reference's 129x129 grid
@@ -258,6 +262,106 @@ def profile_launches(torch, K):
     torch.cuda.synchronize()
 
 
+def all_input_products(torch, K, reps: int = 20):
+    """Products along every input of R(Ma, U, X, V) on the 257x129 pitching workload (one
+    state): the hand-written tangent along (Ma, U, X, V) together and along X alone, the adjoint
+    to all four, against the directional complex step and the complex-step block assembly (the
+    reference's linearize, f90:552-1143); errors against the complex-step blocks."""
+    mach, aoa = 0.6, 0.0
+    amp = np.deg2rad(0.5)
+    alpha, alpha_dot = amp * np.sin(0.4), amp * 0.2 * np.cos(0.4)
+    X, V, U, M, N = make_case(257, 129, 0.04, mach, aoa, alpha, alpha_dot, 20.0, 1)
+    g, p = K.Grid(M, N), K.Params(aoa_deg=aoa)
+    dev = lambda a, dt: torch.as_tensor(np.ascontiguousarray(np.swapaxes(a, -1, -2))).to(dt).cuda()
+    Xd, Vd, Ud = dev(X, torch.float64), dev(V, torch.float64), dev(U[0], torch.float64)
+    rng = np.random.default_rng(2)
+    cr = lambda shape: dev(rng.standard_normal(shape) + 1j * rng.standard_normal(shape),
+                           torch.complex128)
+    dU, dX, dV, w = cr((4, M, N)), cr((2, M + 1, N + 1)), cr((2, M + 1, N + 1)), cr((4, M, N))
+    dMa = complex(rng.standard_normal(), rng.standard_normal())
+    dMa_d = torch.tensor([dMa], dtype=torch.complex128, device="cuda")   # resident: graphs
+    JU, JX, JV, JMa = K.cs_linearize_odes(p, mach, g, Xd, Vd, Ud)
+    out = torch.empty_like(dU)
+    vout = {"states": torch.empty_like(dU), "vertices": torch.empty_like(dX),
+            "velocities": torch.empty_like(dV),
+            "mach": torch.empty(1, dtype=torch.complex128, device="cuda")}
+    allin = dict(dmach=dMa_d, dX=dX, dV=dV, dU=dU)
+    want = ("states", "vertices", "velocities", "mach")
+    t = {
+        "jvp_all_inputs": graph_time_ms(torch, lambda: K.jvp_odes(p, mach, g, Xd, Vd, Ud,
+                                                                  dR=out, **allin), reps),
+        "jvp_vertices": graph_time_ms(torch, lambda: K.jvp_odes(p, mach, g, Xd, Vd, Ud, dX=dX,
+                                                                dR=out), reps),
+        "vjp_all_outputs": graph_time_ms(torch, lambda: K.vjp_odes(p, mach, g, Xd, Vd, Ud, w,
+                                                                   want=want, out=vout), reps),
+        "cs_dir_all_inputs": graph_time_ms(torch, lambda: K.cs_jvp_odes(
+            p, mach, g, Xd, Vd, Ud, dR=out, **allin), reps),
+        "cs_dir_vertices": graph_time_ms(torch, lambda: K.cs_jvp_odes(p, mach, g, Xd, Vd, Ud,
+                                                                      dX=dX, dR=out), reps),
+        "cs_linearize_all": graph_time_ms(torch, lambda: K.cs_linearize_odes(
+            p, mach, g, Xd, Vd, Ud, JU=JU, JX=JX, JV=JV, JMa=JMa), max(2, reps // 10)),
+    }
+    ref_fwd = (K.apply_states_fwd(g, JU, dU) + K.apply_vertices_fwd(g, JX, dX)
+               + K.apply_vertices_fwd(g, JV, dV) + JMa.to(torch.complex128) * dMa)
+    got = K.vjp_odes(p, mach, g, Xd, Vd, Ud, w, want=want)
+    return {
+        "grid": "257x129, pitching", "cells": M * N,
+        "ms": {k: round(v, 5) for k, v in t.items()},
+        "rel_err_jvp_all_vs_cs_blocks": rel(K.jvp_odes(p, mach, g, Xd, Vd, Ud, **allin), ref_fwd),
+        "rel_err_vjp_states": rel(got["states"], K.apply_states_rev(g, JU, w)),
+        "rel_err_vjp_vertices": rel(got["vertices"], K.apply_vertices_rev(g, JX, w)),
+        "rel_err_vjp_velocities": rel(got["velocities"], K.apply_vertices_rev(g, JV, w)),
+        "rel_err_vjp_mach": abs(complex(got["mach"][0]) - complex(
+            (JMa.to(torch.complex128) * w).sum())) / abs(complex((JMa.to(torch.complex128) * w).sum())),
+        "speedup_jvp_all_vs_cs_dir": t["cs_dir_all_inputs"] / t["jvp_all_inputs"],
+        "speedup_jvp_vertices_vs_cs_dir": t["cs_dir_vertices"] / t["jvp_vertices"],
+        "speedup_vjp_all_vs_cs_linearize": t["cs_linearize_all"] / t["vjp_all_outputs"],
+    }
+
+
+def second_order(torch, K, reps: int = 10):
+    """Second-order gradients grad_{Ma,U,X} <dR, dR/dU dU> on the NACA0004 513x257 O-grid (far
+    field 30 chords, Ma 0.5, alpha 1 deg): the exact tangent-over-adjoint product against the
+    reference's method, central differences of complex-step Jacobian products (4 passes of
+    linearize + transposed products per call, core.py:718-760), timed on this GPU; the error
+    is measured against central differences of the hand-written adjoint."""
+    X, V, U, M, N = make_case(513, 257, 0.04, 0.5, 1.0, 0.0, 0.0, 30.0, 1)
+    g, p = K.Grid(M, N), K.Params(aoa_deg=1.0)
+    dev = lambda a, dt: torch.as_tensor(np.ascontiguousarray(np.swapaxes(a, -1, -2))).to(dt).cuda()
+    Xd, Vd, Ud = dev(X, torch.float64), dev(V, torch.float64), dev(U[0], torch.float64)
+    rng = np.random.default_rng(3)
+    cr = lambda shape: rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    w = dev(cr((4, M, N)), torch.complex128)
+    dU = dev(cr((4, M, N)), torch.complex128)
+    want = ("states", "vertices", "mach")
+    t_hvp = graph_time_ms(torch, lambda: K.hvp_odes(p, 0.5, g, Xd, Vd, Ud, w, dU=dU, want=want),
+                          reps)
+    JU, JX, _, JMa = K.cs_linearize_odes(p, 0.5, g, Xd, Vd, Ud, want=want)
+    t_lin = graph_time_ms(torch, lambda: K.cs_linearize_odes(
+        p, 0.5, g, Xd, Vd, Ud, JU=JU, JX=JX, JMa=JMa, want=want), 2)
+    t_rev = graph_time_ms(torch, lambda: (K.apply_states_rev(g, JU, w),
+                                          K.apply_vertices_rev(g, JX, w)), reps)
+    t_ref = 4 * (t_lin + t_rev)
+    H = K.hvp_odes(p, 0.5, g, Xd, Vd, Ud, w, dU=dU, want=want)
+    # central differences of the adjoint along Re dU and Im dU (step ~1e-9 relative: the biased
+    # maximum switches within 1e-8, so larger steps can straddle a switch)
+    hstep = 1e-9 * float(torch.linalg.norm(Ud)) / float(torch.linalg.norm(dU.real))
+    fd = {}
+    for part, d in (("re", dU.real), ("im", dU.imag)):
+        a = K.vjp_odes(p, 0.5, g, Xd, Vd, Ud + hstep * d, w, want=want)
+        b = K.vjp_odes(p, 0.5, g, Xd, Vd, Ud - hstep * d, w, want=want)
+        fd[part] = {k: (a[k] - b[k]) / (2 * hstep) for k in want}
+    err = {k: rel(H[k], fd["re"][k] + 1j * fd["im"][k]) for k in ("states", "vertices")}
+    return {
+        "grid": "513x257", "cells": M * N,
+        "ms": {"tangent_over_adjoint": round(t_hvp, 5), "cs_linearize": round(t_lin, 5),
+               "cs_block_rev_products": round(t_rev, 5),
+               "reference_method_fd_of_cs": round(t_ref, 5)},
+        "speedup_vs_reference_method": t_ref / t_hvp,
+        "rel_err_vs_central_differences_of_adjoint": err,
+    }
+
+
 def multigrid(torch, K, G: int = 64, seed: int = 0, reps: int = 10):
     """64 O-grids of uneven size in one launch against one launch per grid (both replayed from
     a graph, so no host gaps): NACA 00tt with tt in 04..18, A in 65..513 (odd) angular and R in
@@ -393,6 +497,11 @@ def main():
         extra["products_1025x513"] = products(torch, K, 1025, 513, 0.04, 0.5, 0.0, 0.0, 0.0,
                                               30.0, 10)
         progress("products 1025x513 done")
+    extra["products_all_inputs_257x129"] = all_input_products(torch, K)
+    progress("all-input products done")
+    if not args.no_large:
+        extra["second_order_513x257"] = second_order(torch, K)
+        progress("second-order products done")
     if not args.no_gmres:
         extra["gmres_1025x513"] = gmres_steps(torch, K)
         progress("gmres done")

@@ -257,21 +257,19 @@ template <class T, int L, class Src>
 PEA_HD void cell_face_rev(const pea_params& P, const Src& src, const T* uc, int flag, int fc,
                           CellAdj<T, L>& ca, T (*tb)[4][L]) {
     const FaceArgs fa = face_args(flag);
-    T st[4][4];
-    const T* arg[4];
+    // the four arguments by value, chosen with selects: a pointer chosen at run time among
+    // private arrays would put them in scratch memory
+    T arg[4][4];
     bool ghost_at[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int a = fa.a[fc][j];
         ghost_at[j] = a == 9;
-        if (a == 9) {
-            arg[j] = ca.gh;
-        } else if (a == C || a < 0) {
-            arg[j] = uc;
-        } else {
-            src.get(a, st[j]);
-            arg[j] = st[j];
-        }
+        const bool own = a == C || a < 0;
+        T nb_[4];
+        if (!ghost_at[j] && !own) src.get(a, nb_);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) arg[j][v] = ghost_at[j] ? ca.gh[v] : own ? uc[v] : nb_[v];
     }
     const bool diss = fa.a[fc][0] >= 0;
 #pragma unroll

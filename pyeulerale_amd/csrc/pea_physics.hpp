@@ -210,19 +210,17 @@ PEA_HD void cell_ode_src(const pea_params& P, const A& mach, const Src& src, con
     T sum[4];
 #pragma unroll
     for (int fc = 0; fc < 4; ++fc) {
-        T st[4][4];
-        const T* arg[4];
+        // the four arguments by value, chosen with selects: a pointer chosen at run time among
+        // private arrays would put them in scratch memory
+        T arg[4][4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int a = fa.a[fc][k];
-            if (a == 9) {
-                arg[k] = gh;
-            } else if (a == C || a < 0) {
-                arg[k] = uc;
-            } else {
-                src.get(a, st[k]);
-                arg[k] = st[k];
-            }
+            const bool ghost = a == 9, own = a == C || a < 0;
+            T nb[4];
+            if (!ghost && !own) src.get(a, nb);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) arg[k][v] = ghost ? gh[v] : own ? uc[v] : nb[v];
         }
         T f[4];
         face_flux(P, arg[0], arg[1], arg[2], arg[3], g.n[fc], g.gv[fc], fa.a[fc][0] >= 0, f);
