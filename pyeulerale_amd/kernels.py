@@ -267,24 +267,29 @@ _TUNED: dict = {}
 
 
 def autotune_vjp(p: Params, mach: float, g: Grid, X, V, U, reps: int = 5) -> int:
-    """Times every adjoint tile shape that fits this grid and registers the fastest
-    (cached per grid size).  Returns the chosen tile index."""
+    """Times every tile shape of the cell-based adjoint (the products to vertices, velocities
+    and Mach, and to the states of grids narrower than the face-based kernel's tile) that fits
+    this grid and registers the fastest (cached per grid size).  Returns the chosen tile
+    index."""
     key = (g.M, g.N)
     if key in _TUNED:
         return _TUNED[key]
     lib = _lib.load()
     gen = torch.Generator(device=U.device).manual_seed(0)
     w = torch.randn(g.cell_shape, dtype=C128, device=U.device, generator=gen)
-    out = {"states": torch.empty(g.cell_shape, dtype=C128, device=U.device)}
+    out = {"states": torch.empty(g.cell_shape, dtype=C128, device=U.device),
+           "vertices": torch.empty(g.vert_shape, dtype=C128, device=U.device),
+           "velocities": torch.empty(g.vert_shape, dtype=C128, device=U.device),
+           "mach": torch.empty(1, dtype=C128, device=U.device)}
     times = {}
     for t in range(lib.pea_num_tiles()):
         if lib.pea_set_tile(g.M, g.N, t) != 0:
             continue
-        vjp_odes(p, mach, g, X, V, U, w, out=out)
+        vjp_odes(p, mach, g, X, V, U, w, want=tuple(out), out=out)
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         a.record()
         for _ in range(reps):
-            vjp_odes(p, mach, g, X, V, U, w, out=out)
+            vjp_odes(p, mach, g, X, V, U, w, want=tuple(out), out=out)
         b.record()
         b.synchronize()
         times[t] = a.elapsed_time(b) / reps
