@@ -19,7 +19,8 @@ Also reported (same JSON line):
 * second-order gradients (tangent-over-adjoint) at 513x257 against the reference's method,
   central differences of complex-step Jacobian products;
 * batched GMRES Arnoldi-step throughput for 32 shifts at 1025x513;
-* 64 O-grids of uneven size in one multi-grid launch against one launch per grid;
+* 64 O-grids of uneven size in one multi-grid launch against one launch per grid, for the
+  residual and for the whole step (residual, JVP and VJP of every grid, one graph);
 * the response example's steady state and 32-frequency response on the reference's 129x129 grid
   (batched exact resolvent vs one shift at a time vs the reference's SuperLU);
 * the reference's own Fortran (compiled from its source into oracle/_ref) timed on this host's
@@ -390,10 +391,48 @@ def multigrid(torch, K, G: int = 64, seed: int = 0, reps: int = 10):
     t_multi = graph_time_ms(torch, mg.run, reps)
     t_single = graph_time_ms(torch, one_by_one, reps)
     same = all(torch.equal(R, s[6]) for R, s in zip(mg.outputs, singles))
+
+    # the whole step of every grid -- R, the JVP along a seeded complex dU and the VJP of a
+    # seeded complex w -- captured in one graph: one multi-grid launch for R and the JVP and one
+    # for the adjoints, against one JVP (with R) and one VJP launch per grid
+    crand = lambda U: (torch.randn(U.shape, dtype=torch.float64, device="cuda")
+                       + 1j * torch.randn(U.shape, dtype=torch.float64, device="cuda"))
+    torch.manual_seed(seed)
+    dUs = [crand(U) for _, _, _, _, U in items]
+    ws = [crand(U) for _, _, _, _, U in items]
+    mgt = K.MultiGrid([it + (dU,) for it, dU in zip(items, dUs)], tangent=True)
+    mga = K.MultiGrid([it + (w,) for it, w in zip(items, ws)], adjoint=True)
+    adj = [(p, mach, g, X, V, U, w, torch.empty_like(w))
+           for (p, mach, g, X, V, U, _), w in zip(singles, ws)]
+    jv = [(p, mach, g, X, V, U, dU, torch.empty_like(U), torch.empty_like(dU))
+          for (p, mach, g, X, V, U, _), dU in zip(singles, dUs)]
+
+    def vjps():
+        for p, mach, g, X, V, U, w, gU in adj:
+            K.vjp_odes(p, mach, g, X, V, U, w, out={"states": gU})
+
+    def step_multi():
+        mgt.run()
+        mga.run()
+
+    def step_single():
+        for p, mach, g, X, V, U, dU, R, dR in jv:
+            K.jvp_odes(p, mach, g, X, V, U, dU=dU, dR=dR, R=R)
+        vjps()
+    t_step_multi = graph_time_ms(torch, step_multi, reps)
+    t_step_single = graph_time_ms(torch, step_single, reps)
+    same_t = all(torch.equal(R, j[7]) and torch.equal(dR, j[8])
+                 for (R, dR), j in zip(mgt.outputs, jv))
+    same_a = all(torch.equal(gU, a[7]) for gU, a in zip(mga.outputs, adj))
     return {"grids": G, "cells": cells, "tiles": mg.total_tiles,
             "ms_multi_launch": t_multi, "ms_one_launch_per_grid": t_single,
             "speedup": t_single / t_multi, "cells_per_s_multi": cells / (t_multi * 1e-3),
-            "bitwise_equal": bool(same)}
+            "bitwise_equal": bool(same),
+            "step_residual_jvp_vjp": {
+                "ms_two_multi_grid_launches": t_step_multi,
+                "ms_two_launches_per_grid": t_step_single,
+                "speedup": t_step_single / t_step_multi,
+                "bitwise_equal": bool(same_t and same_a)}}
 
 
 def gmres_steps(torch, K, S: int = 32, steps: int = 30):

@@ -84,7 +84,9 @@ int pea_compute_odes_batch(const pea_params *prm, double mach, int M, int N, int
  * pea_multi_prepare: uploads the launch's descriptor table built from the host array `grids`
  *   (G entries) into `workspace` (device, pea_multi_workspace_bytes(G) bytes) and returns the
  *   tile count; it synchronises the stream.  tangent = 0: residuals (R written); tangent = 1:
- *   states-direction JVP (pea_jvp_odes with dU only; dR written, R too when non-NULL).
+ *   states-direction JVP (pea_jvp_odes with dU only; dR written, R too when non-NULL);
+ *   tangent = 2: states adjoint (pea_vjp_odes to the states only), dU holding the seed w and
+ *   dR receiving (dR/dU)^T w; every grid needs N >= 18 (the adjoint tile's width + 3).
  * pea_multi_launch: one launch over the prepared table; it allocates and copies nothing, so it
  *   may be captured in a HIP graph and replayed while the table and arrays stay valid.
  * pea_compute_odes_multi / pea_jvp_odes_multigrid: prepare + launch. */

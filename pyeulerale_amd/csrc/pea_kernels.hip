@@ -1241,9 +1241,9 @@ k_vjp(pea_params P, Ld ld, const double* __restrict__ wseed, double* __restrict_
 // at the end, together with the wall / far-field faces and their ghost states.
 // Requires TN + 3 <= N (no face is reached twice through the periodic seam).
 template <int TM, int TN, int NT>
-__global__ void __launch_bounds__(NT)
-k_face_vjp(pea_params P, FsDir fs, LdReal ld, const double* __restrict__ wseed,
-           double* __restrict__ gU) {
+__device__ __forceinline__ void face_vjp_tile(const pea_params& P, const FsDir& fs,
+                                              const LdReal& ld, const double* __restrict__ wseed,
+                                              double* __restrict__ gU, int m0, int n0) {
     using T = double;
     constexpr int L = 2;
     constexpr int BM = TM + 6, BN = TN + 6, BOX = BM * BN, NC = TM * TN;
@@ -1256,9 +1256,6 @@ k_face_vjp(pea_params P, FsDir fs, LdReal ld, const double* __restrict__ wseed,
     __shared__ T accU[4][L][NC], accP[L][NC], accC[L][NC];
     __shared__ T gU_[2][TN][4][L], gP_[2][TN][L];   // ghost adjoints of the tile's columns
     const Grid g = ld.g;
-    const unsigned tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y,
-                                   gridDim.x * gridDim.y);   // XCD-contiguous tiles
-    const int m0 = (int)(tile % gridDim.x) * TM, n0 = (int)(tile / gridDim.x) * TN;
     const int tid = threadIdx.x;
 
     auto vtx = [&](int mm, int nn, T* x) { ld.vertex(nn * (g.M + 1) + mm, x); };
@@ -1543,6 +1540,16 @@ k_face_vjp(pea_params P, FsDir fs, LdReal ld, const double* __restrict__ wseed,
     }
 }
 
+template <int TM, int TN, int NT>
+__global__ void __launch_bounds__(NT)
+k_face_vjp(pea_params P, FsDir fs, LdReal ld, const double* __restrict__ wseed,
+           double* __restrict__ gU) {
+    const unsigned tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y,
+                                   gridDim.x * gridDim.y);   // XCD-contiguous tiles
+    face_vjp_tile<TM, TN, NT>(P, fs, ld, wseed, gU, (int)(tile % gridDim.x) * TM,
+                              (int)(tile / gridDim.x) * TN);
+}
+
 // Sums per-block complex partials in block order (deterministic).
 __global__ void __launch_bounds__(256)
 k_sum_partials(const double* __restrict__ partial, int count, double* __restrict__ out) {
@@ -1713,6 +1720,25 @@ int launch_vjp_shape(const pea_params& P, const Ld& ld, const double* w, double*
 #endif
 constexpr int kVjpFaceTM = PEA_VJP_TILE_M, kVjpFaceTN = PEA_VJP_TILE_N;
 
+// (dR/dU)^T w of G grids, one workgroup per tile as k_face_fwd_multi: a descriptor's dU holds
+// the seed w and dR receives the product.  Every grid needs N >= the adjoint tile's width + 3.
+__global__ void __launch_bounds__(256)
+k_face_vjp_multi(const GridDesc* __restrict__ desc, int G) {
+    const int t = (int)xcd_tile(blockIdx.x, gridDim.x);
+    int lo = 0, hi = G - 1;   // last grid whose first tile is <= t
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (desc[mid].tile0 <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const GridDesc e = desc[lo];
+    const Grid g{e.d.M, e.d.N, e.d.M * e.d.N, (e.d.M + 1) * (e.d.N + 1)};
+    const int tl = t - e.tile0;
+    face_vjp_tile<kVjpFaceTM, kVjpFaceTN, 256>(
+        e.d.prm, e.fs, LdReal{e.d.X, e.d.V, e.d.U, e.d.mach, g}, e.d.dU, e.d.dR,
+        (tl % e.tiles_m) * kVjpFaceTM, (tl / e.tiles_m) * kVjpFaceTN);
+}
+
 template <class T, class Ld>
 int launch_vjp(const pea_params& P, const Ld& ld, const double* w, double* gU, double* gX,
                double* gV, double* gmach, void* ws, hipStream_t st) {
@@ -1834,7 +1860,9 @@ int pea_multi_prepare(int G, const pea_grid* grids, int tangent, void* workspace
                       int* total_tiles, void* stream) {
     if (G < 1) return fail(PEA_ERR_DIMS, "no grids");
     if (!grids || !workspace || !total_tiles) return fail(PEA_ERR_NULL, "null argument");
-    const int tm = tangent ? kJvpTM : kFwdTM, tn = tangent ? kJvpTN : kFwdTN;
+    if (tangent < 0 || tangent > 2) return fail(PEA_ERR_ARG, "mode %d", tangent);
+    const int tm = tangent == 2 ? kVjpFaceTM : tangent ? kJvpTM : kFwdTM;
+    const int tn = tangent == 2 ? kVjpFaceTN : tangent ? kJvpTN : kFwdTN;
     std::vector<GridDesc> d(G);
     long long total = 0;
     for (int i = 0; i < G; ++i) {
@@ -1842,6 +1870,9 @@ int pea_multi_prepare(int G, const pea_grid* grids, int tangent, void* workspace
         if (int rc = check_dims(q.M, q.N)) return rc;
         if (!q.X || !q.V || !q.U || (tangent ? (!q.dU || !q.dR) : !q.R))
             return fail(PEA_ERR_NULL, "null array in a grid descriptor");
+        if (tangent == 2 && q.N < kVjpFaceTN + 3)
+            return fail(PEA_ERR_DIMS, "multi-grid adjoint needs N >= %d (grid %d: N=%d)",
+                        kVjpFaceTN + 3, i, q.N);
         d[i].d = q;
         d[i].fs = fs_dir(q.prm);
         d[i].tiles_m = (q.M + tm - 1) / tm;
@@ -1862,8 +1893,14 @@ int pea_multi_prepare(int G, const pea_grid* grids, int tangent, void* workspace
 int pea_multi_launch(int G, int total_tiles, int tangent, const void* workspace, void* stream) {
     if (G < 1 || total_tiles < 1) return fail(PEA_ERR_DIMS, "no tiles");
     if (!workspace) return fail(PEA_ERR_NULL, "null argument");
-    const int nt = tangent ? kJvpTM * kJvpTN : kFwdTM * kFwdTN;
+    if (tangent < 0 || tangent > 2) return fail(PEA_ERR_ARG, "mode %d", tangent);
     const GridDesc* d = static_cast<const GridDesc*>(workspace);
+    if (tangent == 2) {
+        k_face_vjp_multi<<<(unsigned)total_tiles, 256, 0, as_stream(stream)>>>(d, G);
+        PEA_LAUNCH_CHECK("k_face_vjp_multi");
+        return PEA_OK;
+    }
+    const int nt = tangent ? kJvpTM * kJvpTN : kFwdTM * kFwdTN;
     if (tangent)
         k_face_fwd_multi<true><<<(unsigned)total_tiles, nt, 0, as_stream(stream)>>>(d, G);
     else

@@ -139,13 +139,18 @@ class MultiGrid:
     """A prepared multi-grid launch over many grids of uneven size.
 
     ``items`` holds (params, mach, X, V, U) per grid in storage layout, plus dU for
-    ``tangent=True`` (the states-direction JVP).  The descriptor table is uploaded once;
-    :meth:`run` is then a single launch that allocates and copies nothing (it may be captured in
-    a CUDA graph) and returns the outputs: the list of R, or of (R, dR).  The input tensors must
-    stay alive and in place while the object is used."""
+    ``tangent=True`` (the states-direction JVP) or the seed w for ``adjoint=True`` (the states
+    adjoint, pea_vjp_odes to the states).  The descriptor table is uploaded once; :meth:`run` is
+    then a single launch that allocates and copies nothing (it may be captured in a CUDA graph)
+    and returns the outputs: the list of R, of (R, dR), or of (dR/dU)^T w.  The input tensors
+    must stay alive and in place while the object is used."""
 
-    def __init__(self, items, tangent: bool = False):
+    def __init__(self, items, tangent: bool = False, adjoint: bool = False):
+        if tangent and adjoint:
+            raise ValueError("tangent or adjoint, not both")
         self.tangent = bool(tangent)
+        self.adjoint = bool(adjoint)
+        self.mode = 2 if adjoint else int(self.tangent)
         self.items = list(items)
         G = len(self.items)
         if G < 1:
@@ -156,24 +161,25 @@ class MultiGrid:
             p, mach, X, V, U = it[:5]
             g = Grid(U.shape[2], U.shape[1])
             g.check_primal(X, V, U)
-            R = torch.empty(g.cell_shape, dtype=F64, device=U.device)
+            R = None if self.adjoint else torch.empty(g.cell_shape, dtype=F64, device=U.device)
             dU = dR = None
-            if self.tangent:
-                dU = _need(it[5], g.cell_shape, C128, "d_states")
+            if self.tangent or self.adjoint:
+                dU = _need(it[5], g.cell_shape, C128, "d_odes" if self.adjoint else "d_states")
                 dR = torch.empty(g.cell_shape, dtype=C128, device=U.device)
             grids[i] = _lib.PeaGrid(g.M, g.N, float(mach), p.c(), X.data_ptr(), V.data_ptr(),
                                     U.data_ptr(), dU.data_ptr() if dU is not None else None,
-                                    R.data_ptr(), dR.data_ptr() if dR is not None else None)
-            self.outputs.append((R, dR) if self.tangent else R)
+                                    R.data_ptr() if R is not None else None,
+                                    dR.data_ptr() if dR is not None else None)
+            self.outputs.append(dR if self.adjoint else (R, dR) if self.tangent else R)
         nbytes = int(_lib.load().pea_multi_workspace_bytes(G))
         self.ws = torch.empty(nbytes, dtype=torch.uint8, device=self.items[0][4].device)
         total = ctypes.c_int(0)
-        _lib.call("pea_multi_prepare", G, grids, int(self.tangent), _ptr(self.ws),
+        _lib.call("pea_multi_prepare", G, grids, self.mode, _ptr(self.ws),
                   ctypes.byref(total), _stream())
         self.G, self.total_tiles = G, total.value
 
     def run(self):
-        _lib.call("pea_multi_launch", self.G, self.total_tiles, int(self.tangent), _ptr(self.ws),
+        _lib.call("pea_multi_launch", self.G, self.total_tiles, self.mode, _ptr(self.ws),
                   _stream())
         return self.outputs
 
@@ -188,6 +194,12 @@ def jvp_odes_multigrid(items):
     """States-direction tangents of many grids in one launch: items (params, mach, X, V, U, dU);
     returns the list of (R, dR)."""
     return MultiGrid(items, tangent=True).run()
+
+
+def vjp_odes_multigrid(items):
+    """States adjoints of many grids in one launch: items (params, mach, X, V, U, w); returns the
+    list of (dR/dU)^T w.  Every grid needs N >= 18 angular cells."""
+    return MultiGrid(items, adjoint=True).run()
 
 
 def compute_forces(g: Grid, X, U, F=None, center=(0.0, 0.0), coef=None):

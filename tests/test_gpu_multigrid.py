@@ -1,6 +1,7 @@
 """GPU tests of the multi-grid launch: many O-grids of uneven size (different thickness, size,
 Mach, angle of attack, JST constants and pitch motion) in one launch give bitwise the
-results of one single-grid call per grid, for the residual and for the states tangent."""
+results of one single-grid call per grid, for the residual, the states tangent and the states
+adjoint."""
 
 from __future__ import annotations
 
@@ -65,6 +66,25 @@ def test_multigrid_tangent_is_bitwise_single(grids):
         ref = K.jvp_odes(p, mach, g, X, V, U, dU=dU, R=R1)
         assert torch.equal(dR, ref), (g.M, g.N)
         assert torch.equal(R, R1)
+
+
+def test_multigrid_adjoint_is_bitwise_single(grids):
+    from pyeulerale_amd import kernels as K
+    from pyeulerale_amd._lib import PeaError
+    wide = [it for it in grids if it[4].shape[1] >= 18]   # the adjoint tile needs N >= 18
+    print(f"[multigrid] adjoint, {len(wide)} grids", flush=True)
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    items = [(p, mach, X, V, U, torch.randn(U.shape, dtype=torch.complex128, device="cuda",
+                                            generator=gen)) for (p, mach, X, V, U) in wide]
+    outs = K.vjp_odes_multigrid(items)
+    for (p, mach, X, V, U, w), gU in zip(items, outs):
+        g = K.Grid(U.shape[2], U.shape[1])
+        ref = K.vjp_odes(p, mach, g, X, V, U, w)["states"]
+        assert torch.equal(gU, ref), (g.M, g.N)
+    narrow = next(it for it in grids if it[4].shape[1] < 18)
+    w = torch.zeros(narrow[4].shape, dtype=torch.complex128, device="cuda")
+    with pytest.raises(PeaError):
+        K.vjp_odes_multigrid([narrow + (w,)])
 
 
 def test_multigrid_rejects_bad_descriptors(grids):
