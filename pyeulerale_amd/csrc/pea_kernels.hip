@@ -1005,6 +1005,24 @@ struct AdjIO<Dual<2>> {
         out[2 * e + 1] = a[0].d[1] + a[1].d[0];
     }
 };
+// The same product in two passes with one seed lane each (PART 0: real part of the seed,
+// written; PART 1: imaginary part, added): half the adjoint state per thread.
+template <int PART>
+struct HvpIO {
+    static constexpr int L = 1;
+    PEA_HD static void seed(const double* w, long e, Dual<2>* s) {
+        s[0] = dconst<2>(w[2 * e + PART]);
+    }
+    PEA_HD static void put(double* out, long e, const Dual<2>* a) {
+        if (PART == 0) {
+            out[2 * e] = a[0].d[0];
+            out[2 * e + 1] = a[0].d[1];
+        } else {
+            out[2 * e] = out[2 * e] - a[0].d[1];
+            out[2 * e + 1] = out[2 * e + 1] + a[0].d[0];
+        }
+    }
+};
 
 // Reverse of the surface forces F_n = p(U_{0,n}) normal(X_{0,n} -> X_{0,n+1}) (f90:1156-1174).
 // gU on the wall ring (zero elsewhere), one thread per cell; T = double gives the adjoint,
@@ -1075,11 +1093,12 @@ k_vjp_forces_vertices(Ld ld, const double* __restrict__ wF, double* __restrict__
 // and independent of the tile shape.
 constexpr int kVjpThreads = 256;
 
-template <class T, int TM, int TN, bool WU, bool WX, bool WV, bool WMA, class Ld>
+template <class T, int TM, int TN, bool WU, bool WX, bool WV, bool WMA, class Ld,
+          class IO = AdjIO<T>>
 __global__ void __launch_bounds__(kVjpThreads)
 k_vjp(pea_params P, Ld ld, const double* __restrict__ wseed, double* __restrict__ gU,
       double* __restrict__ gX, double* __restrict__ gV, double* __restrict__ partial) {
-    constexpr int L = AdjIO<T>::L;
+    constexpr int L = IO::L;
     constexpr int BM = TM + 4, BN = TN + 4, NBOX = BM * BN;
     constexpr int ROUNDS = (NBOX + kVjpThreads - 1) / kVjpThreads;
     constexpr int NVX = (WX || WV) ? (TM + 1) * (TN + 1) : 1;
@@ -1133,7 +1152,7 @@ k_vjp(pea_params P, Ld ld, const double* __restrict__ wseed, double* __restrict_
             ld.state(c, uc);
             T ws[4][L];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) AdjIO<T>::seed(wseed, (long)v * g.C + c, ws[v]);
+            for (int v = 0; v < 4; ++v) IO::seed(wseed, (long)v * g.C + c, ws[v]);
             cell_adj_init<T, L>(P, ld.machv(), src, uc, x, xw, flag, ws, WX && gX, ca);
         }
 #pragma unroll
@@ -1200,7 +1219,7 @@ k_vjp(pea_params P, Ld ld, const double* __restrict__ wseed, double* __restrict_
             if (mt < g.M && nt < g.N)
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
-                    AdjIO<T>::put(gU, (long)v * g.C + nt * g.M + mt, accU[t][v]);
+                    IO::put(gU, (long)v * g.C + nt * g.M + mt, accU[t][v]);
         }
     if (WX || WV)
         for (int t = tid; t < NVX; t += kVjpThreads) {
@@ -1211,8 +1230,8 @@ k_vjp(pea_params P, Ld ld, const double* __restrict__ wseed, double* __restrict_
 #pragma unroll
                 for (int d = 0; d < 2; ++d) {
                     const long e = (long)d * g.P + nv * (g.M + 1) + mv;
-                    if (WX && gX) AdjIO<T>::put(gX, e, accX[t][d]);
-                    if (WV && gV) AdjIO<T>::put(gV, e, accV[t][d]);
+                    if (WX && gX) IO::put(gX, e, accX[t][d]);
+                    if (WV && gV) IO::put(gV, e, accV[t][d]);
                 }
         }
     if (WMA) {
@@ -1226,7 +1245,7 @@ k_vjp(pea_params P, Ld ld, const double* __restrict__ wseed, double* __restrict_
             __syncthreads();
         }
         if (tid == 0 && partial)
-            AdjIO<T>::put(partial, blockIdx.y * gridDim.x + blockIdx.x, red[0]);
+            IO::put(partial, blockIdx.y * gridDim.x + blockIdx.x, red[0]);
     }
 }
 
@@ -1696,19 +1715,33 @@ int pick_tile(const Grid& g, bool small) {
     return best;
 }
 
-template <class T, int TM, int TN, class Ld>
-int launch_vjp_shape(const pea_params& P, const Ld& ld, const double* w, double* gU, double* gX,
-                     double* gV, double* partial, hipStream_t st) {
+template <class T, int TM, int TN, class Ld, class IO>
+int launch_vjp_pass(const pea_params& P, const Ld& ld, const double* w, double* gU, double* gX,
+                    double* gV, double* partial, hipStream_t st) {
     const dim3 grid((ld.g.M + TM - 1) / TM, (ld.g.N + TN - 1) / TN);
     if (gU && !gX && !gV && !partial) {
-        k_vjp<T, TM, TN, true, false, false, false, Ld>
+        k_vjp<T, TM, TN, true, false, false, false, Ld, IO>
             <<<grid, kVjpThreads, 0, st>>>(P, ld, w, gU, gX, gV, partial);
     } else {
-        k_vjp<T, TM, TN, true, true, true, true, Ld>
+        k_vjp<T, TM, TN, true, true, true, true, Ld, IO>
             <<<grid, kVjpThreads, 0, st>>>(P, ld, w, gU, gX, gV, partial);
     }
     PEA_LAUNCH_CHECK("k_vjp");
     return PEA_OK;
+}
+
+// The adjoint (T = double) in one pass; the tangent-over-adjoint (T = Dual<2>) in two passes of
+// one seed lane each, the second adding into the first's outputs (and block partials).
+template <class T, int TM, int TN, class Ld>
+int launch_vjp_shape(const pea_params& P, const Ld& ld, const double* w, double* gU, double* gX,
+                     double* gV, double* partial, hipStream_t st) {
+    if constexpr (std::is_same<T, double>::value) {
+        return launch_vjp_pass<T, TM, TN, Ld, AdjIO<T>>(P, ld, w, gU, gX, gV, partial, st);
+    } else {
+        if (int rc = launch_vjp_pass<T, TM, TN, Ld, HvpIO<0>>(P, ld, w, gU, gX, gV, partial, st))
+            return rc;
+        return launch_vjp_pass<T, TM, TN, Ld, HvpIO<1>>(P, ld, w, gU, gX, gV, partial, st);
+    }
 }
 
 // Tile of the face-based adjoint (cells per workgroup of 256 threads): its TM (TN + 3) angular
