@@ -146,18 +146,21 @@ PEA_HD void face_flux2_lds(const pea_params& P, const St& st, const T& pLL, cons
             side_coefs(P, lamR, lamL, sensor(pRR, pR, pL), sensor(pR, pL, pLL), e2R, e4R);
         }
     }
+    const G hvn = vn / 2.0;   // (L + R)/2 vn == (L + R)(vn/2): halving is exact
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const T Lv = st(1, v), Rv = st(2, v);
         const T phL = v == 0 ? Lv * qL : v == 3 ? (Lv + pL) * qL : Lv * qL + n[v - 1] * pL;
         const T phR = v == 0 ? Rv * qR : v == 3 ? (Rv + pR) * qR : Rv * qR + n[v - 1] * pR;
         const T cen = (phL + phR) / 2.0;
-        const T ale = (Lv + Rv) / 2.0 * vn;
+        const T ale = (Lv + Rv) * hvn;
         T a = cen, b = -cen;
         if (dissL || dissR) {
-            const T LLv = st(0, v), RRv = st(3, v);
-            if (dissL) a = a - (e2L * (Rv - Lv) - e4L * (RRv - 3.0 * Rv + 3.0 * Lv - LLv));
-            if (dissR) b = b - (e2R * (Lv - Rv) - e4R * (LLv - 3.0 * Lv + 3.0 * Rv - RRv));
+            // second and fourth differences seen from L; R's side sees their negatives (the
+            // reference sums R's in its own order, a rounding apart)
+            const T d2 = Rv - Lv, d4 = st(3, v) - 3.0 * Rv + 3.0 * Lv - st(0, v);
+            if (dissL) a = a - (e2L * d2 - e4L * d4);
+            if (dissR) b = b + (e2R * d2 - e4R * d4);
         }
         outL(v, a - ale);
         outR(v, b + ale);
@@ -168,10 +171,12 @@ PEA_HD void face_flux2_lds(const pea_params& P, const St& st, const T& pLL, cons
 // Reverse of side_dissipation with seed db on D: accumulates into the adjoints of the side's
 // ll, l, r, rr states (sb*), of its two wave integrals (lamb_l, lamb_r) and of the four
 // pressures entering its sensors (pb_ll, pb_l, pb_r, pb_rr).  Switch decisions are recomputed.
+// d2, d4: the face's second and fourth state differences seen from its L side (r - l and
+// rr - 3r + 3l - ll); flip = true for the R side, which sees their negatives.
 template <class T, int L>
 PEA_HD void side_dissipation_rev(const pea_params& P, const T& lam_l, const T& lam_r,
                                  const T& pll, const T& pl, const T& pr, const T& prr,
-                                 const T* ll, const T* l, const T* r, const T* rr,
+                                 const T* d2, const T* d4, bool flip,
                                  const T (*db)[L], T (*sbll)[L], T (*sbl)[L], T (*sbr)[L],
                                  T (*sbrr)[L], T* lamb_l, T* lamb_r, T* pbll, T* pbl, T* pbr,
                                  T* pbrr) {
@@ -193,9 +198,9 @@ PEA_HD void side_dissipation_rev(const pea_params& P, const T& lam_l, const T& l
         T e2b = adj_zero<T>(), e4b = adj_zero<T>();
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const T d = db[v][k];
-            e2b = e2b + d * (r[v] - l[v]);
-            e4b = e4b - d * (rr[v] - 3.0 * r[v] + 3.0 * l[v] - ll[v]);
+            const T d = db[v][k], x = flip ? -d : d;
+            e2b = e2b + x * d2[v];
+            e4b = e4b - x * d4[v];
             const T t2 = d * e2, tq = d * e4;
             sbr[v][k] = sbr[v][k] + t2 + 3.0 * tq;
             sbl[v][k] = sbl[v][k] - t2 - 3.0 * tq;
@@ -277,6 +282,12 @@ PEA_HD void face_flux2_rev(const pea_params& P, const T* LL, const T* Lc, const 
     }
     if (dissL || dissR) {
         const G sn = ssqrt(n[0] * n[0] + n[1] * n[1]);
+        T d2[4], d4[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            d2[v] = R[v] - Lc[v];
+            d4[v] = RR[v] - 3.0 * R[v] + 3.0 * Lc[v] - LL[v];
+        }
         if (dissL) {
             const double sL = sign1(qL), sR = sign1(qR);
             const T lamL = sabs(qL) + cL * sn, lamR = sabs(qR) + cR * sn;
@@ -287,7 +298,7 @@ PEA_HD void face_flux2_rev(const pea_params& P, const T* LL, const T* Lc, const 
 #pragma unroll
                 for (int v = 0; v < 4; ++v) db[v][k] = -fbL[v][k];
             }
-            side_dissipation_rev<T, L>(P, lamL, lamR, pLL, pL, pR, pRR, LL, Lc, R, RR, db,
+            side_dissipation_rev<T, L>(P, lamL, lamR, pLL, pL, pR, pRR, d2, d4, false, db,
                                        a.u[0], a.u[1], a.u[2], a.u[3], lbl, lbr, a.p[0],
                                        a.p[1], a.p[2], a.p[3]);
 #pragma unroll
@@ -310,7 +321,7 @@ PEA_HD void face_flux2_rev(const pea_params& P, const T* LL, const T* Lc, const 
 #pragma unroll
                 for (int v = 0; v < 4; ++v) db[v][k] = -fbR[v][k];
             }
-            side_dissipation_rev<T, L>(P, lamR, lamL, pRR, pR, pL, pLL, RR, R, Lc, LL, db,
+            side_dissipation_rev<T, L>(P, lamR, lamL, pRR, pR, pL, pLL, d2, d4, true, db,
                                        a.u[3], a.u[2], a.u[1], a.u[0], lbr, lbl, a.p[3],
                                        a.p[2], a.p[1], a.p[0]);
 #pragma unroll
