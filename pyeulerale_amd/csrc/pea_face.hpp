@@ -23,22 +23,6 @@ PEA_HD T sensor(const T& pa, const T& pb, const T& pc) {
     return qdiv(sabs(pa - 2.0 * pb + pc), pa + 2.0 * pb + pc);
 }
 
-// Dissipation of one side (f90:207-215): returns D with f -= D.
-//   lam_self/lam_other: wave integrals of this side's l and r;  s1 = sensor(ll, l, r),
-//   s2 = sensor(l, r, rr);  l, r, ll, rr: states as seen from this side.
-template <class T>
-PEA_HD void side_dissipation(const pea_params& P, const T& lam_l, const T& lam_r, const T& s1,
-                             const T& s2, const T* ll, const T* l, const T* r, const T* rr,
-                             T* D) {
-    const T lam = maxb(lam_l, lam_r);
-    const T s = maxb(s1, s2);
-    const T e2 = P.k2 * s * lam;
-    const T e4 = maxb(lit<T>(0.0), P.k4 * lam - P.c4 * e2);
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
-        D[v] = e2 * (r[v] - l[v]) - e4 * (rr[v] - 3.0 * r[v] + 3.0 * l[v] - ll[v]);
-}
-
 // |q| whose tangent is s * dq: the derivative convention of abs_c (f90:97-102) for a value
 // q' = -q that R's side computes with its own normal; only the sign of an exact zero differs.
 PEA_HD double abs_signed(double q, double) { return fabs(q); }
@@ -51,12 +35,6 @@ PEA_HD Dual<K> abs_signed(const Dual<K>& q, double s) {
     return r;
 }
 
-// Both fluxes of one face.  n is L's normal of the face (pointing to R), nR is R's own normal
-// of it: the same edge traversed the other way, so nR == -n except for the signs of exact
-// zeros, which decide the tangent of |n.j/rho| in R's evaluation (abs_c, f90:97-102).
-// p* and c* are the cells' pressures and sound speeds; ll/rr (and their pressures) are only
-// read by a dissipative side.  f_L and f_R are the contributions to the residual sums of L and
-// R, those of face_flux() evaluated from each side.
 // Switch coefficients of one side (f90:207-213): e2 and e4 with D = e2 d2 - e4 d4.
 template <class T>
 PEA_HD void side_coefs(const pea_params& P, const T& lam_l, const T& lam_r, const T& s1,
@@ -67,55 +45,16 @@ PEA_HD void side_coefs(const pea_params& P, const T& lam_l, const T& lam_r, cons
     e4 = maxb(lit<T>(0.0), P.k4 * lam - P.c4 * e2);
 }
 
-template <class T, class G>
-PEA_HD void face_flux2(const pea_params& P, const T* LL, const T* L, const T* R, const T* RR,
-                       const T& pLL, const T& pL, const T& pR, const T& pRR, const T& cL,
-                       const T& cR, const G* n, const G* nR, const G* gv, bool dissL,
-                       bool dissR, T* fL, T* fR) {
-    // scalar coefficients first, then one component at a time: only the stencil states, the
-    // coefficients and the current component are live (the tangent types are register-bound)
-    const T qL = (n[0] * L[1] + n[1] * L[2]) / L[0];
-    const T qR = (n[0] * R[1] + n[1] * R[2]) / R[0];
-    const G vn = gv[0] * n[0] + gv[1] * n[1];
-    T e2L = lit<T>(0.0), e4L = lit<T>(0.0), e2R = lit<T>(0.0), e4R = lit<T>(0.0);
-    if (dissL || dissR) {
-        const G sn = ssqrt(n[0] * n[0] + n[1] * n[1]);
-        if (dissL) {
-            const T lamL = sabs(qL) + cL * sn;
-            const T lamR = sabs(qR) + cR * sn;
-            side_coefs(P, lamL, lamR, sensor(pLL, pL, pR), sensor(pL, pR, pRR), e2L, e4L);
-        }
-        if (dissR) {
-            // R's view: q' = (nR . j) / rho = -q up to the sign of an exact zero
-            const double sL = -sign1(val(nR[0]) * val(L[1]) + val(nR[1]) * val(L[2]));
-            const double sR = -sign1(val(nR[0]) * val(R[1]) + val(nR[1]) * val(R[2]));
-            const T lamL = abs_signed(qL, sL) + cL * sn;
-            const T lamR = abs_signed(qR, sR) + cR * sn;
-            side_coefs(P, lamR, lamL, sensor(pRR, pR, pL), sensor(pR, pL, pLL), e2R, e4R);
-        }
-    }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        const T phL = v == 0 ? L[0] * qL
-                             : v == 3 ? (L[3] + pL) * qL : L[v] * qL + n[v - 1] * pL;
-        const T phR = v == 0 ? R[0] * qR
-                             : v == 3 ? (R[3] + pR) * qR : R[v] * qR + n[v - 1] * pR;
-        const T cen = (phL + phR) / 2.0;
-        const T ale = (L[v] + R[v]) / 2.0 * vn;
-        T a = cen;
-        if (dissL) a = a - (e2L * (R[v] - L[v]) - e4L * (RR[v] - 3.0 * R[v] + 3.0 * L[v] - LL[v]));
-        fL[v] = a - ale;
-        T b = -cen;
-        if (dissR) b = b - (e2R * (L[v] - R[v]) - e4R * (LL[v] - 3.0 * L[v] + 3.0 * R[v] - RR[v]));
-        fR[v] = b + ale;
-    }
-}
-
-// face_flux2 with the stencil read through an accessor and the results handed to sinks:
-// st(slot, v) returns component v of slot 0..3 = ll, l, r, rr; outL(v, f) and outR(v, f) take
-// the two sides' fluxes.  Only the scalar coefficients and one component of the stencil are
-// live at a time (the states sit in LDS), which keeps the tangent types within registers.
-// Same expressions, same results as face_flux2.
+// Both fluxes of one face.  n is L's normal of the face (pointing to R), nR is R's own normal
+// of it: the same edge traversed the other way, so nR == -n except for the signs of exact
+// zeros, which decide the tangent of |n.j/rho| in R's evaluation (abs_c, f90:97-102).
+// p* and c* are the cells' pressures and sound speeds; ll/rr (and their pressures) are only
+// read by a dissipative side.  f_L and f_R are the contributions to the residual sums of L and
+// R, those of face_flux() evaluated from each side.
+// The stencil is read through an accessor and the results handed to sinks: st(slot, v) returns
+// component v of slot 0..3 = ll, l, r, rr; outL(v, f) and outR(v, f) take the two sides'
+// fluxes.  Only the scalar coefficients and one component of the stencil are live at a time
+// (the states sit in LDS), which keeps the tangent types within registers.
 template <class T, class G, class St, class OutL, class OutR>
 PEA_HD void face_flux2_lds(const pea_params& P, const St& st, const T& pLL, const T& pL,
                            const T& pR, const T& pRR, const T& cL, const T& cR, const G* n,
@@ -246,7 +185,7 @@ PEA_HD void face_adj_zero(FaceAdj<T, L>& a) {
         }
 }
 
-// Reverse of face_flux2 with seeds fbL on f_L and fbR on f_R (the faces' shares of the two
+// Reverse of face_flux2_lds with seeds fbL on f_L and fbR on f_R (the faces' shares of the two
 // cells' residual seeds, -w/area).  Accumulates into `a`; pressures and sound speeds are
 // treated as inputs (their adjoints go to a.p / a.c).
 template <class T, class G, int L>

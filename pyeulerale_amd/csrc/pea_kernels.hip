@@ -319,12 +319,13 @@ k_jvp(pea_params P, Ld ld, double* __restrict__ dR, double* __restrict__ R) {
 
 // ------------------------------------------------------------------ face-based forward
 // Residual (T = double) or residual + tangent (T = Dual<K>) of a TM x TN tile of cells, one
-// thread per cell.  Every face touching the tile is evaluated once (face_flux2): a thread owns
-// its cell's f and o faces, hands the opposite side's flux to its neighbour through LDS, and
-// the tile's first column / first row also evaluate the faces shared with the neighbouring
-// tiles.  Per-cell pressures and sound speeds of the tile grown by two cells are computed once
-// into LDS; the wall and far-field ghosts sit in the grown rows m = -1 and m = M.  Each cell
-// then sums its four face fluxes in the reference's order (b, f, i, o; f90:395-433).
+// thread per cell.  Every face touching the tile is evaluated once (face_flux2_lds): a thread
+// owns its cell's f and o faces and hands the other side's flux to the neighbouring cell (a
+// lane shuffle within the wave, LDS across waves); the faces the tile shares with its
+// neighbouring tiles are evaluated by the first wave.  Per-cell pressures and sound speeds of
+// the tile grown by two cells are computed once into LDS; the wall and far-field ghosts sit in
+// the grown rows m = -1 and m = M.  Each cell sums its four face fluxes in a fixed order for its
+// position in the tile (the reference's b, f, i, o of f90:395-433, a rounding apart).
 // blockIdx.z selects one of several states on the same grid (consecutive 4*C arrays).
 // A value of scalar type T (double, Dual<K>, Cx: plain aggregates of doubles) stored in LDS as
 // structure of arrays: component k of element i at base[k * stride + i].
@@ -1512,7 +1513,7 @@ __device__ __forceinline__ void face_vjp_tile(const pea_params& P, const FsDir& 
             T nr[2], gv[2];
             if (side == 0) edge(0, n, 0, n + 1, nr, gv);           // wall face bi -> fi
             else edge(m + 1, n + 1, m + 1, n, nr, gv);             // far field fo -> bo
-            // central face (cell, ghost) seen from the cell: f_L of face_flux2 with L = cell
+            // central face (cell, ghost) seen from the cell: f_L of face_flux2_lds with L = cell
             FaceAdj<T, L> a;
             face_adj_zero(a);
             const T zero[4][L] = {{-0.0, -0.0}, {-0.0, -0.0}, {-0.0, -0.0}, {-0.0, -0.0}};

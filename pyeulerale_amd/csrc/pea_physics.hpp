@@ -113,14 +113,6 @@ PEA_HD void wall_ghost(const T* u, const G* n, const G* gv, T* g) {
     g[3] = u[3];
 }
 
-// Everything one cell's residual depends on.
-template <class T>
-struct Stencil {
-    T u[9][4];   // states, Slot order
-    T x[4][2];   // vertices, Vtx order
-    T w[4][2];   // vertex velocities, Vtx order
-};
-
 // Cell geometry derived from the vertices: area, outward normals and face velocities.
 template <class T>
 struct CellGeom {
@@ -229,22 +221,6 @@ PEA_HD void cell_ode_src(const pea_params& P, const A& mach, const Src& src, con
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) R[v] = -(sum[v] / g.area);
-}
-
-template <class T>
-struct StencilSrc {
-    const Stencil<T>& s;
-    PEA_HD void get(int slot, T* u) const {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) u[v] = s.u[slot][v];
-    }
-};
-
-template <class T>
-PEA_HD void cell_ode(const pea_params& P, const T& mach, const Stencil<T>& s, int flag, T* R) {
-    CellGeom<T> g;
-    cell_geometry(s.x, s.w, g);
-    cell_ode_src(P, mach, StencilSrc<T>{s}, s.u[C], g, flag, R);
 }
 
 }  // namespace pea
