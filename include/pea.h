@@ -218,7 +218,9 @@ int pea_block_multi(int M, int N, int S, const double *JU, const double *shifts,
                     int transposed, const double *x, double *y, void *stream);
 
 /* Radial-line preconditioner: block LU of the block-pentadiagonal matrix of each radial line
- * (blocks ii, i, c - sI, o, oo) for every shift; F needs pea_line_factor_bytes(M, N, S). */
+ * (blocks ii, i, c - sI, o, oo) for every shift; F needs pea_line_factor_bytes(M, N, S).  F holds
+ * all the solve uses: pea_line_solve ignores JU (may be null) and `transposed`, which the
+ * factorisation already applied; r, x and F 16-byte aligned, x may alias r. */
 long long pea_line_factor_bytes(int M, int N, int S);
 int pea_line_factor(int M, int N, int S, const double *JU, const double *shifts,
                     int transposed, double *F, void *stream);
@@ -255,11 +257,22 @@ int pea_gmres_rotate(int S, int R, int j, double *H, double *cs, double *sn, dou
                      double *est, void *stream);
 
 /* Gram-Schmidt projections: h[s*bsh + i] = sum_k conj(V[s*bsv + i*ldv + k]) w[s*bsw + k] for
- * i < q, k < n, s < S (complex; strides in complex elements).  K is split into nb chunks
- * summed in a fixed order; P is a workspace of S*nb*q complex values. */
+ * i < q, k < n, s < S (complex; strides in complex elements; V and w 16-byte aligned).  K is
+ * split into nb chunks summed in a fixed order; P is a workspace of S*nb*q complex values. */
 int pea_multidot(int S, int n, int q, const double *V, long long ldv, long long bsv,
                  const double *w, long long bsw, int nb, double *P, double *h, long long bsh,
                  void *stream);
+
+/* Gram-Schmidt update: w[s*bsw + k] -= sum_{i<q} V[s*bsv + i*ldv + k] h[s*bsh + i] (complex,
+ * strides in complex elements, V and w 16-byte aligned), each row of V read once.  mode 0: update
+ * only; mode 1: also out[s*bso + i] = sum_k conj(V[s*bsv + i*ldv + k]) w_new[s*bsw + k] for i < q
+ * (the next projections, same chunked order as pea_multidot); mode 2: also out[s*bso] =
+ * |w_new[s]|^2 (imaginary part 0).  P: workspace of S*nb*q complex values (modes 1 and 2).
+ * The reference solves each shift directly (spsolve, core.py:562 and 596) and has no
+ * counterpart; this is part of the batched GMRES that stands in for those solves. */
+int pea_gs_update(int S, int n, int q, const double *V, long long ldv, long long bsv,
+                  const double *h, long long bsh, double *w, long long bsw, int mode, int nb,
+                  double *P, double *out, long long bso, void *stream);
 
 /* Block-Jacobi preconditioner: inverses of the 4x4 diagonal blocks (J_cc - s I);
  * F holds S*16*M*N complex values. */

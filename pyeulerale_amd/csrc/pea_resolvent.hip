@@ -7,7 +7,8 @@
 // * k_line_factor / k_line_solve: radial-line preconditioner.  For every angular index n and
 //   shift s the block-pentadiagonal matrix of the radial line (blocks ii, i, c - sI, o, oo) is
 //   factorised by block LU (partial pivoting inside each 4x4 diagonal block) and solved.
-//   One thread per (line, shift), m sequential.  The transposed variant factorises the line of
+//   Factorisation: one thread per (line, shift), m sequential; solve: four lanes per (line,
+//   shift), one per row of the 4x4 blocks.  The transposed variant factorises the line of
 //   (dR/dU)^T - conj(s) I for the adjoint solve.
 // * k_jacobi_factor / k_jacobi_solve: block-Jacobi preconditioner (4x4 diagonal blocks).
 // * k_ring_band / k_ring_couple: pieces of the exact block-banded LU over radial rings.  A ring
@@ -17,11 +18,15 @@
 //   block diagonal in n (blocks ii, i, o, oo).  k_ring_band writes the dense ring block for all
 //   shifts; k_ring_couple forms Z = beta Z + alpha C Y with a coupling C, or C itself as a dense
 //   matrix when Y is null.  The dense eliminations between them are library GEMM/LU calls.
+// * k_multidot_partial / k_gs_update / k_multidot_final: Gram-Schmidt of batched GMRES.  A
+//   projection pass reads the basis once; the update pass subtracts the projections and, reading
+//   the basis rows only once more, forms the next projections or the norm of the result.
 #include <hip/hip_runtime.h>
 
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <cstdint>
 #include <cstdio>
 #include <mutex>
 
@@ -132,10 +137,27 @@ __device__ __forceinline__ B4 load_jblock(const double* JU, int C, int cell, int
     return r;
 }
 
-// factor storage: per (shift, m, slot, i, j) a plane over n:  F[(((sh*M + m)*4 + slot)*16
-// + 4i + j)*N + n], slots 0 = L2, 1 = L1, 2 = Dinv, 3 = U1
+// factor storage: lines in groups of kLineGroup (the lines one wave of the solve holds); per
+// (shift, group) the steps m one after another, each with its 4 slots x 16 elements x
+// kLineGroup lines:  F[((((sh*NG + g)*M + m)*4 + slot)*16 + 4i + j)*kLineGroup + n % kLineGroup]
+// with g = n / kLineGroup, slots 0 = L2, 1 = L1, 2 = Dinv, 3 = U1.  A wave thus streams through
+// one contiguous region as it walks the line, instead of touching 64 planes spread over the
+// whole array at every step.
+constexpr int kLineGroup = 16;
+__host__ __device__ __forceinline__ long line_groups(int N) {
+    return (N + kLineGroup - 1) / kLineGroup;
+}
 __device__ __forceinline__ long fidx(int M, int N, int sh, int m, int slot, int ij, int n) {
-    return ((((long)sh * M + m) * 4 + slot) * 16 + ij) * N + n;
+    const long g = n / kLineGroup, nl = n % kLineGroup;
+    return (((((long)sh * line_groups(N) + g) * M + m) * 4 + slot) * 16 + ij) * kLineGroup + nl;
+}
+// after the S shifts' complex factors (in doubles): the real block U2 = (m, m+2) of the system,
+// the same for every shift, element (i, j) of line n at U2[((g*M + m)*16 + 4i + j)*kLineGroup
+// + n % kLineGroup]
+__device__ __forceinline__ long u2idx(int M, int N, int S, int m, int ij, int n) {
+    const long base = 2L * S * line_groups(N) * M * 4 * 16 * kLineGroup;
+    const long g = n / kLineGroup, nl = n % kLineGroup;
+    return base + (((long)g * M + m) * 16 + ij) * kLineGroup + nl;
 }
 __device__ __forceinline__ void store_b(double* F, int M, int N, int sh, int m, int slot, int n,
                                         const B4& b) {
@@ -205,73 +227,199 @@ k_line_factor(int M, int N, int S, const double* __restrict__ JU, const double* 
             if (m >= 1) U1 = bsub(U1, bmul(L1, line_block(JU, M, N, n, m - 1, 2, tr, shift)));
             store_b(F, M, N, s, m, 3, n, U1);
         }
+        if (s == 0 && m + 2 < M) {
+            const B4 U2 = line_block(JU, M, N, n, m, 2, tr, shift);
+#pragma unroll
+            for (int ij = 0; ij < 16; ++ij) F[u2idx(M, N, S, m, ij, n)] = U2.a[ij / 4][ij % 4].r;
+        }
         store_b(F, M, N, s, m, 2, n, binv(D));
     }
 }
 
+// Lane J of each quad of lanes, broadcast to the quad (DPP quad_perm [J, J, J, J]).
+template <int J>
+__device__ __forceinline__ double quad_bcast(double x) {
+    constexpr int ctrl = J * 0x55;
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, ctrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), ctrl, 0xF, 0xF, false);
+    return __longlong_as_double(
+        (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo));
+}
+// the 4-vector whose component i lane i of the quad holds
+__device__ __forceinline__ void quad_gather(Z mine, Z* y) {
+    y[0] = Z{quad_bcast<0>(mine.r), quad_bcast<0>(mine.i)};
+    y[1] = Z{quad_bcast<1>(mine.r), quad_bcast<1>(mine.i)};
+    y[2] = Z{quad_bcast<2>(mine.r), quad_bcast<2>(mine.i)};
+    y[3] = Z{quad_bcast<3>(mine.r), quad_bcast<3>(mine.i)};
+}
+
+// Orders the LDS accesses of a wave's lanes: what one lane wrote, another lane reads after this.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // x = (line LU)^-1 r for every line and shift.  r and x: S consecutive complex cell fields
-// (4 planes of M x N each); x may alias r.
-__global__ void __launch_bounds__(64)
-k_line_solve(int M, int N, int S, const double* __restrict__ JU, const double* __restrict__ F,
-             int tr, const double* r, double* x) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = blockIdx.y;
-    if (n >= N || s >= S) return;
-    const long C = (long)M * N, base = (long)s * 4 * C;
-    auto ld = [&](const double* a, int m, Z* y) {
+// (4 planes of M x N each); x may alias r.  Four lanes per (line, shift): lane i holds row i of
+// the 4-vectors and of the factor blocks, and the whole vectors of the two previous steps are
+// shared in the quad by DPP broadcasts.  A wave holds kLineGroup lines.
+// * The vectors run along m, which is the fastest index of the cell fields, so a lane's step
+//   would touch 16 B of its own cache line: the wave moves them through LDS in blocks of
+//   kLineBlock steps instead, whole lines at a time.
+// * The factor loads of a step do not depend on the recurrence: they are issued kLineAhead
+//   steps early into a ring of register sets (the block loop is unrolled, so every set is
+//   addressed at compile time).
+constexpr int kLineThreads = 256;
+constexpr int kLineAhead = 4;
+constexpr int kLineBlock = 8;        // 8 steps x 16 B: one 128 B line per (line, row)
+constexpr int kLinePad = kLineBlock + 1;
+
+struct LineFwd {   // operands of one forward step
+    Z l2[4], l1[4];
+};
+struct LineBwd {   // operands of one backward step
+    Z u1[4], di[4];
+    double u2[4];
+};
+
+__global__ void __launch_bounds__(kLineThreads)
+k_line_solve(int M, int N, int S, const double* __restrict__ F, const double* r, double* x) {
+    static_assert(kLineThreads % 64 == 0 && 64 / 4 == kLineGroup, "a wave holds one line group");
+    static_assert(kLineBlock % kLineAhead == 0, "ring slots repeat within a block");
+    __shared__ double2 sv[kLineThreads / 64][64][kLinePad];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int n0 = (blockIdx.x * kLineThreads + wv * 64) / 4;   // first line of the wave
+    const int n = n0 + lane / 4, i = lane & 3, s = blockIdx.y;
+    if (n0 >= N || s >= S) return;   // whole waves; a wave's lines past N compute unused values
+    const long C = (long)M * N;
+    const double2* F2 = reinterpret_cast<const double2*>(F);
+    const double2* r2 = reinterpret_cast<const double2*>(r);
+    double2* x2 = reinterpret_cast<double2*>(x);
+    double2(*buf)[kLinePad] = sv[wv];
+    const Z zero = {0.0, 0.0};
+    // block transfer: element e = lane + 64k of the wave's 64 x kLineBlock block is step e % 8
+    // of pair e / 8, the pair being lane (line, row) = (e / 32, e / 8 % 4) of the compute phase
+    auto vload = [&](const double2* a, int m0) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const long e = base + v * C + (long)n * M + m;
-            y[v] = Z{a[2 * e], a[2 * e + 1]};
+        for (int k = 0; k < kLineBlock; ++k) {
+            const int e = lane + 64 * k, p = e / kLineBlock, q = e % kLineBlock;
+            const int pn = n0 + p / 4, m = m0 + q;
+            if (pn < N && m < M) buf[p][q] = a[(long)s * 4 * C + (long)(p & 3) * C + (long)pn * M + m];
         }
     };
-    auto st = [&](double* a, int m, const Z* y) {
+    auto vstore = [&](double2* a, int m0) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const long e = base + v * C + (long)n * M + m;
-            a[2 * e] = y[v].r;
-            a[2 * e + 1] = y[v].i;
+        for (int k = 0; k < kLineBlock; ++k) {
+            const int e = lane + 64 * k, p = e / kLineBlock, q = e % kLineBlock;
+            const int pn = n0 + p / 4, m = m0 + q;
+            if (pn < N && m < M) a[(long)s * 4 * C + (long)(p & 3) * C + (long)pn * M + m] = buf[p][q];
         }
     };
-    auto mv = [&](const B4& b, const Z* y, Z* acc) {   // acc -= b y
+    auto frow = [&](int m, int slot, Z* a) {   // row i of factor block (m, slot)
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i] = zsub(acc[i], zmul(b.a[i][j], y[j]));
+        for (int j = 0; j < 4; ++j) {
+            const double2 v = F2[fidx(M, N, s, m, slot, 4 * i + j, n)];
+            a[j] = Z{v.x, v.y};
+        }
     };
     // forward: y_m = r_m - L2 y_{m-2} - L1 y_{m-1}  (y stored in x)
-    Z y1[4], y2[4];
-    for (int m = 0; m < M; ++m) {
-        Z y[4];
-        ld(r, m, y);
-        if (m >= 2) mv(load_b(F, M, N, s, m, 0, n), y2, y);
-        if (m >= 1) mv(load_b(F, M, N, s, m, 1, n), y1, y);
-        st(x, m, y);
+    {
+        Z y1[4] = {zero, zero, zero, zero}, y2[4] = {zero, zero, zero, zero};
+        auto load = [&](int m, LineFwd& f) {
+            if (m < M) {
+                if (m >= 2) frow(m, 0, f.l2);
+                if (m >= 1) frow(m, 1, f.l1);
+            }
+        };
+        LineFwd ring[kLineAhead];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            y2[v] = y1[v];
-            y1[v] = y[v];
+        for (int k = 0; k < kLineAhead; ++k) load(k, ring[k]);
+        for (int m0 = 0; m0 < M; m0 += kLineBlock) {
+            vload(r2, m0);
+            wave_sync();
+#pragma unroll
+            for (int k = 0; k < kLineBlock; ++k) {
+                const int m = m0 + k;
+                LineFwd& f = ring[k % kLineAhead];
+                if (m < M) {
+                    const double2 rv = buf[lane][k];
+                    Z acc = {rv.x, rv.y};
+                    if (m >= 2)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) acc = zsub(acc, zmul(f.l2[j], y2[j]));
+                    if (m >= 1)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) acc = zsub(acc, zmul(f.l1[j], y1[j]));
+                    buf[lane][k] = make_double2(acc.r, acc.i);
+                    Z y[4];
+                    quad_gather(acc, y);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        y2[j] = y1[j];
+                        y1[j] = y[j];
+                    }
+                }
+                load(m + kLineAhead, f);
+            }
+            wave_sync();
+            vstore(x2, m0);
+            wave_sync();
         }
     }
-    // backward: x_m = Dinv (y_m - U1 x_{m+1} - U2 x_{m+2}),  U2 = block (m, m+2)
-    const Z zero = {0.0, 0.0};
-    Z x1[4], x2[4];
-    for (int m = M - 1; m >= 0; --m) {
-        Z y[4];
-        ld(x, m, y);
-        if (m + 1 < M) mv(load_b(F, M, N, s, m, 3, n), x1, y);
-        if (m + 2 < M) mv(line_block(JU, M, N, n, m, 2, tr, zero), x2, y);
-        Z xm[4] = {zero, zero, zero, zero};
-        const B4 Di = load_b(F, M, N, s, m, 2, n);
+    // backward: x_m = Dinv (y_m - U1 x_{m+1} - U2 x_{m+2}),  U2 = block (m, m+2) of the system
+    {
+        Z x1[4] = {zero, zero, zero, zero}, xx2[4] = {zero, zero, zero, zero};
+        auto load = [&](int m, LineBwd& b) {
+            if (m >= 0 && m < M) {
+                if (m + 1 < M) frow(m, 3, b.u1);
+                if (m + 2 < M)
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+                    for (int j = 0; j < 4; ++j) b.u2[j] = F[u2idx(M, N, S, m, 4 * i + j, n)];
+                frow(m, 2, b.di);
+            }
+        };
+        const int mt = (M - 1) / kLineBlock * kLineBlock;   // first step of the top block
+        // step m = m0 + kLineBlock - 1 - k of a block uses ring slot m % kLineAhead
+        LineBwd ring[kLineAhead];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) xm[i] = zfma(Di.a[i][j], y[j], xm[i]);
-        st(x, m, xm);
+        for (int k = 0; k < kLineAhead; ++k)
+            load(mt + kLineBlock - 1 - k, ring[(kLineBlock - 1 - k) % kLineAhead]);
+        for (int m0 = mt; m0 >= 0; m0 -= kLineBlock) {
+            vload(x2, m0);
+            wave_sync();
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            x2[v] = x1[v];
-            x1[v] = xm[v];
+            for (int k = 0; k < kLineBlock; ++k) {
+                const int q = kLineBlock - 1 - k, m = m0 + q;
+                LineBwd& b = ring[q % kLineAhead];
+                if (m < M) {
+                    const double2 yv = buf[lane][q];
+                    Z acc = {yv.x, yv.y};
+                    if (m + 1 < M)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) acc = zsub(acc, zmul(b.u1[j], x1[j]));
+                    if (m + 2 < M)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) acc = zsub(acc, zmul(Z{b.u2[j], 0.0}, xx2[j]));
+                    Z yq[4];
+                    quad_gather(acc, yq);
+                    Z xm = zero;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) xm = zfma(b.di[j], yq[j], xm);
+                    buf[lane][q] = make_double2(xm.r, xm.i);
+                    Z xq[4];
+                    quad_gather(xm, xq);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        xx2[j] = x1[j];
+                        x1[j] = xq[j];
+                    }
+                }
+                load(m - kLineAhead, b);
+            }
+            wave_sync();
+            vstore(x2, m0);
+            wave_sync();
         }
     }
 }
@@ -548,55 +696,184 @@ k_gmres_rotate(int S, int R, int j, double* __restrict__ Hd, double* __restrict_
 // deterministic, no atomics, no memset.
 constexpr int kDotThreads = 256;
 
-__global__ void __launch_bounds__(kDotThreads)
-k_multidot_partial(int n, int q, const double* __restrict__ V, long ldv, long bsv,
-                   const double* __restrict__ w, long bsw, int nb, double* __restrict__ P) {
-    const int b = blockIdx.x, s = blockIdx.y, tid = threadIdx.x;
-    const long chunk = ((long)n + nb - 1) / nb;
-    const long k0 = (long)b * chunk, k1 = min((long)n, k0 + chunk);
-    __shared__ double red_r[kDotThreads], red_i[kDotThreads];
-    const double* ws = w + 2 * (long)s * bsw;
-    for (int i = 0; i < q; ++i) {
-        const double* v = V + 2 * ((long)s * bsv + (long)i * ldv);
-        double ar = 0.0, ai = 0.0;
-        for (long k = k0 + tid; k < k1; k += kDotThreads) {
-            const double vr = v[2 * k], vi = v[2 * k + 1], xr = ws[2 * k], xi = ws[2 * k + 1];
-            ar = fma(vr, xr, fma(vi, xi, ar));    // conj(v) * x
-            ai = fma(vr, xi, fma(-vi, xr, ai));
-        }
-        red_r[tid] = ar;
-        red_i[tid] = ai;
-        __syncthreads();
-        for (int h = kDotThreads / 2; h > 0; h >>= 1) {
-            if (tid < h) {
-                red_r[tid] += red_r[tid + h];
-                red_i[tid] += red_i[tid + h];
+// Sums acc[i], i < nout, over the workgroup (a butterfly in each wave, then the waves in order)
+// into out[2i], out[2i + 1].
+template <int QMAX>
+__device__ __forceinline__ void block_sums(const Z* acc, int nout, double (*s_red)[2 * QMAX],
+                                           double* __restrict__ out) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+    for (int i = 0; i < QMAX; ++i) {
+        if (i < nout) {
+            double ar = acc[i].r, ai = acc[i].i;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                ar += __shfl_xor(ar, off);
+                ai += __shfl_xor(ai, off);
             }
-            __syncthreads();
+            if (lane == 0) {
+                s_red[wv][2 * i] = ar;
+                s_red[wv][2 * i + 1] = ai;
+            }
         }
-        if (tid == 0) {
-            const long e = ((long)s * nb + b) * q + i;
-            P[2 * e] = red_r[0];
-            P[2 * e + 1] = red_i[0];
-        }
-        __syncthreads();
+    }
+    __syncthreads();
+    if (tid < 2 * nout) {
+        double t = 0.0;
+#pragma unroll
+        for (int u = 0; u < kDotThreads / 64; ++u) t += s_red[u][tid];
+        out[tid] = t;
     }
 }
 
+// Stage 1: workgroup (chunk b, shift s) reads each element of w once and the q <= QMAX rows of V
+// at it, and writes the chunk's q sums to P[((s*nb + b)*pq + i)] (complex; pq >= q is the row
+// count of the whole projection when rows come in slices, P then offset to the slice).
+template <int QMAX>
+__global__ void __launch_bounds__(kDotThreads)
+k_multidot_partial(int n, int q, const double* __restrict__ V, long ldv, long bsv,
+                   const double* __restrict__ w, long bsw, int nb, int pq, double* __restrict__ P) {
+    __shared__ double s_red[kDotThreads / 64][2 * QMAX];
+    const int b = blockIdx.x, s = blockIdx.y, tid = threadIdx.x;
+    const long chunk = ((long)n + nb - 1) / nb;
+    const long k0 = (long)b * chunk, k1 = min((long)n, k0 + chunk);
+    const double2* Vs = reinterpret_cast<const double2*>(V) + (long)s * bsv;
+    const double2* ws = reinterpret_cast<const double2*>(w) + (long)s * bsw;
+    Z acc[QMAX];
+#pragma unroll
+    for (int i = 0; i < QMAX; ++i) acc[i] = Z{0.0, 0.0};
+#pragma unroll 1
+    for (long k = k0 + tid; k < k1; k += kDotThreads) {
+        const double2 x = ws[k];
+#pragma unroll
+        for (int i = 0; i < QMAX; ++i)
+            if (i < q) {   // conj(v) * x
+                const double2 v = Vs[(long)i * ldv + k];
+                acc[i].r = fma(v.x, x.x, fma(v.y, x.y, acc[i].r));
+                acc[i].i = fma(v.x, x.y, fma(-v.y, x.x, acc[i].i));
+            }
+    }
+    block_sums<QMAX>(acc, q, s_red, P + 2 * ((long)s * nb + b) * pq);
+}
+
+// Stage 2: one wave per output (s, i).  Lane l adds chunks l, l + 64, ...; a butterfly over the
+// wave adds the lanes.  Both orders are fixed, so the sum is deterministic.
 __global__ void __launch_bounds__(64)
 k_multidot_final(int S, int q, int nb, const double* __restrict__ P, double* __restrict__ h,
                  long bsh) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x;
     if (t >= S * q) return;
-    const int s = t / q, i = t - s * q;
+    const int s = t / q, i = t - s * q, lane = threadIdx.x;
     double ar = 0.0, ai = 0.0;
-    for (int b = 0; b < nb; ++b) {
+    for (int b = lane; b < nb; b += 64) {
         const long e = ((long)s * nb + b) * q + i;
         ar += P[2 * e];
         ai += P[2 * e + 1];
     }
-    h[2 * ((long)s * bsh + i)] = ar;
-    h[2 * ((long)s * bsh + i) + 1] = ai;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        ar += __shfl_xor(ar, off);
+        ai += __shfl_xor(ai, off);
+    }
+    if (lane == 0) {
+        h[2 * ((long)s * bsh + i)] = ar;
+        h[2 * ((long)s * bsh + i) + 1] = ai;
+    }
+}
+
+// Fused Gram-Schmidt pass: w[s] -= sum_{i<q} V[s][i] h[s][i], reading each row of V once, and in
+// the same pass (mode 1) the partial projections conj(V[s][i]) . w_new of every row, or (mode 2)
+// the partial squared norm |w_new|^2, one value per (chunk, s) for k_multidot_final.  Mode 0
+// updates only.  The chunking of K is that of k_multidot_partial.  q <= QMAX: the rows of one
+// element and their projections stay in registers.
+template <int QMAX>
+__global__ void __launch_bounds__(kDotThreads)
+k_gs_update(int n, int q, const double* __restrict__ V, long ldv, long bsv,
+            const double* __restrict__ h, long bsh, double* __restrict__ w, long bsw, int nb,
+            int mode, double* __restrict__ P) {
+    __shared__ double s_h[2 * QMAX];
+    __shared__ double s_red[kDotThreads / 64][2 * QMAX];
+    const int b = blockIdx.x, s = blockIdx.y, tid = threadIdx.x;
+    if (tid < 2 * q) s_h[tid] = h[2 * (long)s * bsh + tid];
+    __syncthreads();
+    const long chunk = ((long)n + nb - 1) / nb;
+    const long k0 = (long)b * chunk, k1 = min((long)n, k0 + chunk);
+    const double2* Vs = reinterpret_cast<const double2*>(V) + (long)s * bsv;
+    double2* ws = reinterpret_cast<double2*>(w) + (long)s * bsw;
+    Z acc[QMAX];
+#pragma unroll
+    for (int i = 0; i < QMAX; ++i) acc[i] = Z{0.0, 0.0};
+#pragma unroll 1
+    for (long k = k0 + tid; k < k1; k += kDotThreads) {
+        const double2 x0 = ws[k];
+        Z x = {x0.x, x0.y};
+        Z v[QMAX];
+#pragma unroll
+        for (int i = 0; i < QMAX; ++i) {
+            if (i < q) {
+                const double2 t = Vs[(long)i * ldv + k];
+                v[i] = Z{t.x, t.y};
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < QMAX; ++i) {   // x -= v_i h_i, rows in order
+            if (i < q) {
+                const double hr = s_h[2 * i], hi = s_h[2 * i + 1];
+                x = Z{fma(-v[i].r, hr, fma(v[i].i, hi, x.r)), fma(-v[i].r, hi, fma(-v[i].i, hr, x.i))};
+            }
+        }
+        ws[k] = make_double2(x.r, x.i);
+        if (mode == 1) {
+#pragma unroll
+            for (int i = 0; i < QMAX; ++i)
+                if (i < q) {   // conj(v) * x
+                    acc[i].r = fma(v[i].r, x.r, fma(v[i].i, x.i, acc[i].r));
+                    acc[i].i = fma(v[i].r, x.i, fma(-v[i].i, x.r, acc[i].i));
+                }
+        } else if (mode == 2) {
+            acc[0].r = fma(x.r, x.r, fma(x.i, x.i, acc[0].r));
+        }
+    }
+    if (mode == 0) return;
+    const int nout = mode == 1 ? q : 1;
+    block_sums<QMAX>(acc, nout, s_red, P + 2 * ((long)s * nb + b) * nout);
+}
+
+template <int QMAX>
+void launch_gs_update(int S, int n, int q, const double* V, long ldv, long bsv, const double* h,
+                      long bsh, double* w, long bsw, int nb, int mode, double* P, hipStream_t st) {
+    k_gs_update<QMAX><<<dim3(nb, S), kDotThreads, 0, st>>>(n, q, V, ldv, bsv, h, bsh, w, bsw, nb,
+                                                           mode, P);
+}
+constexpr int kGsMaxRows = 32;
+
+template <int QMAX>
+void launch_multidot(int S, int n, int q, const double* V, long ldv, long bsv, const double* w,
+                     long bsw, int nb, int pq, double* P, hipStream_t st) {
+    k_multidot_partial<QMAX><<<dim3(nb, S), kDotThreads, 0, st>>>(n, q, V, ldv, bsv, w, bsw, nb,
+                                                                  pq, P);
+}
+// projections of q rows (any count): slices of at most kGsMaxRows rows, then the chunk sums
+void multidot(int S, int n, int q, const double* V, long ldv, long bsv, const double* w, long bsw,
+              int nb, double* P, double* h, long bsh, hipStream_t st) {
+    for (int i0 = 0; i0 < q; i0 += kGsMaxRows) {
+        const int qs = q - i0 < kGsMaxRows ? q - i0 : kGsMaxRows;
+        const double* Vi = V + 2 * (long)i0 * ldv;
+        double* Pi = P + 2 * (long)i0;
+        if (qs <= 4) launch_multidot<4>(S, n, qs, Vi, ldv, bsv, w, bsw, nb, q, Pi, st);
+        else if (qs <= 8) launch_multidot<8>(S, n, qs, Vi, ldv, bsv, w, bsw, nb, q, Pi, st);
+        else if (qs <= 16) launch_multidot<16>(S, n, qs, Vi, ldv, bsv, w, bsw, nb, q, Pi, st);
+        else launch_multidot<kGsMaxRows>(S, n, qs, Vi, ldv, bsv, w, bsw, nb, q, Pi, st);
+    }
+    k_multidot_final<<<S * q, 64, 0, st>>>(S, q, nb, P, h, bsh);
+}
+
+void gs_update_slice(int S, int n, int q, const double* V, long ldv, long bsv, const double* h,
+                     long bsh, double* w, long bsw, int nb, int mode, double* P, hipStream_t st) {
+    if (q <= 4) launch_gs_update<4>(S, n, q, V, ldv, bsv, h, bsh, w, bsw, nb, mode, P, st);
+    else if (q <= 8) launch_gs_update<8>(S, n, q, V, ldv, bsv, h, bsh, w, bsw, nb, mode, P, st);
+    else if (q <= 16) launch_gs_update<16>(S, n, q, V, ldv, bsv, h, bsh, w, bsw, nb, mode, P, st);
+    else launch_gs_update<kGsMaxRows>(S, n, q, V, ldv, bsv, h, bsh, w, bsw, nb, mode, P, st);
 }
 
 // One rocBLAS handle per device for the dense factorisations, created on first use.
@@ -637,7 +914,7 @@ extern "C" {
 const char* pea_resolvent_last_error(void) { return g_rerr; }
 
 long long pea_line_factor_bytes(int M, int N, int S) {
-    return (long long)S * M * 4 * 16 * N * 16;
+    return (long long)line_groups(N) * kLineGroup * M * 16 * (S * 4 * 16 + 8);
 }
 
 int pea_line_factor(int M, int N, int S, const double* JU, const double* shifts, int transposed,
@@ -652,9 +929,14 @@ int pea_line_factor(int M, int N, int S, const double* JU, const double* shifts,
 int pea_line_solve(int M, int N, int S, const double* JU, const double* F, int transposed,
                    const double* r, double* x, void* stream) {
     if (M < 4 || N < 5 || S < 1 || S > 65535) return rfail(PEA_ERR_DIMS, "bad dimensions");
-    if (!JU || !F || !r || !x) return rfail(PEA_ERR_NULL, "null argument");
-    k_line_solve<<<dim3((N + 63) / 64, S), 64, 0, (hipStream_t)stream>>>(M, N, S, JU, F,
-                                                                        transposed, r, x);
+    if (!F || !r || !x) return rfail(PEA_ERR_NULL, "null argument");
+    if (((uintptr_t)F | (uintptr_t)r | (uintptr_t)x) & 15)
+        return rfail(PEA_ERR_DIMS, "F, r and x must be 16-byte aligned");
+    // the factorisation stored the system's blocks (transposed or not): the solve is the same
+    (void)transposed;
+    (void)JU;
+    k_line_solve<<<dim3((4 * N + kLineThreads - 1) / kLineThreads, S), kLineThreads, 0,
+                   (hipStream_t)stream>>>(M, N, S, F, r, x);
     return hipGetLastError() == hipSuccess ? PEA_OK : rfail(PEA_ERR_HIP, "k_line_solve");
 }
 
@@ -758,10 +1040,36 @@ int pea_multidot(int S, int n, int q, const double* V, long long ldv, long long 
                  void* stream) {
     if (S < 1 || n < 1 || q < 1 || nb < 1 || S > 65535) return rfail(PEA_ERR_DIMS, "bad dimensions");
     if (!V || !w || !P || !h) return rfail(PEA_ERR_NULL, "null argument");
-    hipStream_t st = (hipStream_t)stream;
-    k_multidot_partial<<<dim3(nb, S), kDotThreads, 0, st>>>(n, q, V, ldv, bsv, w, bsw, nb, P);
-    k_multidot_final<<<(S * q + 63) / 64, 64, 0, st>>>(S, q, nb, P, h, bsh);
+    if (((uintptr_t)V | (uintptr_t)w) & 15) return rfail(PEA_ERR_DIMS, "V and w must be 16-byte aligned");
+    multidot(S, n, q, V, ldv, bsv, w, bsw, nb, P, h, bsh, (hipStream_t)stream);
     return hipGetLastError() == hipSuccess ? PEA_OK : rfail(PEA_ERR_HIP, "k_multidot");
+}
+
+int pea_gs_update(int S, int n, int q, const double* V, long long ldv, long long bsv,
+                  const double* h, long long bsh, double* w, long long bsw, int mode, int nb,
+                  double* P, double* out, long long bso, void* stream) {
+    if (S < 1 || n < 1 || q < 1 || nb < 1 || S > 65535) return rfail(PEA_ERR_DIMS, "bad dimensions");
+    if (mode < 0 || mode > 2) return rfail(PEA_ERR_DIMS, "mode must be 0, 1 or 2");
+    if (!V || !h || !w) return rfail(PEA_ERR_NULL, "null argument");
+    if (mode != 0 && (!P || !out)) return rfail(PEA_ERR_NULL, "null workspace or output");
+    if (((uintptr_t)V | (uintptr_t)w) & 15) return rfail(PEA_ERR_DIMS, "V and w must be 16-byte aligned");
+    hipStream_t st = (hipStream_t)stream;
+    // rows in slices of at most kGsMaxRows; the norm comes with the last slice, the projections
+    // of more rows than one slice holds from a separate multi-dot over the updated w
+    for (int i0 = 0; i0 < q; i0 += kGsMaxRows) {
+        const int qs = q - i0 < kGsMaxRows ? q - i0 : kGsMaxRows;
+        const bool last = i0 + qs == q;
+        const int md = !last ? 0 : (mode == 1 && q > kGsMaxRows) ? 0 : mode;
+        gs_update_slice(S, n, qs, V + 2 * (long)i0 * ldv, ldv, bsv, h + 2 * (long)i0, bsh, w, bsw,
+                        nb, md, P, st);
+    }
+    if (mode == 1 && q > kGsMaxRows) {
+        multidot(S, n, q, V, ldv, bsv, w, bsw, nb, P, out, bso, st);
+    } else if (mode != 0) {
+        const int nout = mode == 1 ? q : 1;
+        k_multidot_final<<<S * nout, 64, 0, st>>>(S, nout, nb, P, out, bso);
+    }
+    return hipGetLastError() == hipSuccess ? PEA_OK : rfail(PEA_ERR_HIP, "k_gs_update");
 }
 
 }  // extern "C"

@@ -135,22 +135,19 @@ class BatchedGMRES:
         w.copy_(V[:, j].reshape(w.shape))
         self.psolve(w, z)
         self.apply(z, w)
-        wf = w.reshape(S, n, 1)
-        Vj = V[:, :j + 1]
         q = j + 1
-        # CGS2: two passes of h = V^H w, w -= V h.  The projections use pea_multidot (K split
-        # into chunks); a library GEMM of this shape (<= 31 x 1 output, K = 4C) does not split K.
+        # CGS2: two passes of h = V^H w, w -= V h, in three passes over the basis rows:
+        # h1 = V^H w (pea_multidot: K split into chunks, which a library GEMM of this shape,
+        # <= 31 x 1 output and K = 4C, does not do); w -= V h1 together with h2 = V^H w; and
+        # w -= V h2 together with |w|^2 (pea_gs_update).
         h1, h2 = st["h1"], st["h2"]
         self._dots(st, V, R + 1, q, w, h1)
-        wf -= torch.bmm(h1[:, None, :q], Vj).transpose(1, 2)
-        self._dots(st, V, R + 1, q, w, h2)
-        wf -= torch.bmm(h2[:, None, :q], Vj).transpose(1, 2)
-        h = (h1[:, :q] + h2[:, :q])[:, :, None]
-        self._dots(st, w, 1, 1, w, st["nrm"])
+        self._gs_update(st, V, R + 1, q, h1, w, 1, h2)
+        self._gs_update(st, V, R + 1, q, h2, w, 2, st["nrm"])
         hn = st["nrm"][:, 0].real.sqrt()
-        V[:, j + 1] = wf[:, :, 0] / hn.clamp_min(1e-300)[:, None]
+        torch.div(w.reshape(S, n), hn.clamp_min(1e-300)[:, None], out=V[:, j + 1])
         H = st["H"]
-        H[:, :j + 1, j] = h[:, :, 0]
+        torch.add(h1[:, :q], h2[:, :q], out=H[:, :q, j])
         H[:, j + 1, j] = hn.to(C128)
         _lib.call("pea_gmres_rotate", S, R, j, _p(H), _p(st["cs"]), _p(st["sn"]), _p(st["g"]),
                   _p(st["est"]), K._stream())
@@ -161,6 +158,14 @@ class BatchedGMRES:
         n = st["n"]
         _lib.call("pea_multidot", self.S, n, q, _p(V), n, rows * n, _p(w), n, st["nb"],
                   _p(st["P"]), _p(out), out.shape[1], K._stream())
+
+    def _gs_update(self, st, V, rows, q, h, w, mode, out):
+        """w[s] -= sum_{i<q} V[s, i] h[s, i]; mode 1 also gives out[s, i] = V[s, i]^H w_new,
+        mode 2 out[s, 0] = |w_new[s]|^2 (V with ``rows`` rows per shift)."""
+        n = st["n"]
+        _lib.call("pea_gs_update", self.S, n, q, _p(V), n, rows * n, _p(h), h.shape[1], _p(w), n,
+                  mode, st["nb"], _p(st["P"]), _p(out), 0 if out is None else out.shape[1],
+                  K._stream())
 
     def _capture(self, st):
         """One CUDA graph per step index j, captured once (after an eager cycle has warmed the
@@ -246,8 +251,11 @@ class BatchedGMRES:
             Hj = torch.triu(st["H"][:, :j_done, :j_done])
             Y = torch.linalg.solve_triangular(Hj, st["g"][:, :j_done, None], upper=True)
             Y[torch.as_tensor(converged, device=dev)] = 0
-            upd = torch.bmm(Y.transpose(1, 2).contiguous(), st["V"][:, :j_done]).reshape(shape)
-            self.psolve(upd.contiguous(), st["z"])
+            # V y as 0 - V (-y): one pass over the basis rows
+            upd = torch.zeros(shape, dtype=C128, device=dev)
+            negY = (-Y[:, :, 0]).contiguous()
+            self._gs_update(st, st["V"], R + 1, j_done, negY, upd, 0, None)
+            self.psolve(upd, st["z"])
             x += st["z"]
             restarts += 1
         self.info = GMRESInfo(iterations=iters, residuals=resid, converged=converged,

@@ -48,6 +48,99 @@ def test_block_multi_matches_single_products(sd):
             assert rel(y[k].cpu().numpy(), ref.cpu().numpy()) <= 1e-13
 
 
+@pytest.mark.parametrize("adjoint", [False, True])
+def test_line_preconditioner_solves_each_radial_line(sd, adjoint):
+    """pea_line_factor/pea_line_solve against dense solves of every radial line's system in numpy:
+    the line's blocks ii, i, c, o, oo of dR/dU minus s I (transposed, minus conj(s) I, for the
+    adjoint), for several shifts at once."""
+    from pyeulerale_amd import _lib
+    from pyeulerale_amd import kernels as K
+    from pyeulerale_amd.gmres import _p
+    g = sd._grid
+    M, N = g.M, g.N
+    JU = sd._get_blocks()["U"]
+    J = JU.cpu().numpy()                      # (slot, i, j, n, m)
+    shifts = np.array([complex(0.5, 1.0), complex(0.0, 3.0), complex(-0.2, 0.05)])
+    S = len(shifts)
+    rng = np.random.default_rng(1)
+    b = crandn(rng, (S, 4, N, M))
+    slot = {-2: 6, -1: 5, 0: 0, 1: 7, 2: 8}
+    ref = np.empty_like(b)
+    for n in range(N):
+        A = np.zeros((4 * M, 4 * M))
+        for m in range(M):
+            for off, sl in slot.items():
+                if 0 <= m + off < M:
+                    A[4 * m:4 * m + 4, 4 * (m + off):4 * (m + off) + 4] = J[sl, :, :, n, m]
+        if adjoint:
+            A = A.T
+        for k, s in enumerate(shifts):
+            sh = np.conj(s) if adjoint else s
+            rhs = b[k, :, n, :].T.reshape(-1)            # line vector, index 4m + v
+            ref[k, :, n, :] = np.linalg.solve(A - sh * np.eye(4 * M), rhs).reshape(M, 4).T
+    sh_d = torch.as_tensor(shifts).cuda()
+    F = torch.empty(int(_lib.load().pea_line_factor_bytes(M, N, S)) // 8, dtype=torch.float64,
+                    device="cuda")
+    _lib.call("pea_line_factor", M, N, S, _p(JU), _p(sh_d), int(adjoint), _p(F), K._stream())
+    bd = torch.as_tensor(b).cuda()
+    x = torch.empty_like(bd)
+    _lib.call("pea_line_solve", M, N, S, _p(JU), _p(F), int(adjoint), _p(bd), _p(x), K._stream())
+    assert rel(x.cpu().numpy(), ref) <= 1e-11
+    # in place (x aliasing r): the same result
+    _lib.call("pea_line_solve", M, N, S, _p(JU), _p(F), int(adjoint), _p(bd), _p(bd),
+              K._stream())
+    assert torch.equal(bd, x)
+
+
+@pytest.mark.parametrize("q", [1, 5, 16, 23, 40])
+def test_gs_update_matches_torch(q):
+    """The projections V^H w, and the fused Gram-Schmidt pass, against the same sums in torch:
+    the update w -= V h (mode 0), with
+    the next projections V^H w (mode 1, and past 32 rows through the separate multi-dot) or the
+    squared norm (mode 2); rows beyond q and other shifts' data untouched, repeat calls bitwise
+    equal."""
+    from pyeulerale_amd import _lib
+    from pyeulerale_amd import kernels as K
+    from pyeulerale_amd.gmres import _p
+    print(f"[resolvent] gs_update q={q}", flush=True)
+    c128 = torch.complex128
+    S, n, rows, ld = 3, 10007, 41, 45
+    rng = np.random.default_rng(q)
+    V = torch.as_tensor(crandn(rng, (S, rows, n))).cuda()
+    w0 = torch.as_tensor(crandn(rng, (S, n))).cuda()
+    h = torch.as_tensor(crandn(rng, (S, ld))).cuda()
+    nb = 7
+    P = torch.empty((S, nb, rows), dtype=c128, device="cuda")
+    ref = w0 - (V[:, :q] * h[:, :q, None]).sum(1)
+    dots = (V[:, :q].conj() * ref[:, None]).sum(-1)
+    nrm = (ref.abs() ** 2).sum(-1)
+    proj = torch.full((S, ld), 7.0, dtype=c128, device="cuda")
+    _lib.call("pea_multidot", S, n, q, _p(V), n, rows * n, _p(w0), n, nb, _p(P), _p(proj), ld,
+              K._stream())
+    ref_proj = (V[:, :q].conj() * w0[:, None]).sum(-1)
+    assert rel(proj[:, :q].cpu().numpy(), ref_proj.cpu().numpy()) <= 1e-13
+    assert (proj[:, q:] == 7.0).all()
+    for mode in (0, 1, 2):
+        outs = []
+        for _ in range(2):
+            w = w0.clone()
+            out = torch.full((S, ld), 7.0, dtype=c128, device="cuda")
+            _lib.call("pea_gs_update", S, n, q, _p(V), n, rows * n, _p(h), ld, _p(w), n, mode,
+                      nb, _p(P), _p(out), ld, K._stream())
+            outs.append((w.clone(), out.clone()))
+        assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])
+        w, out = outs[0]
+        assert rel(w.cpu().numpy(), ref.cpu().numpy()) <= 1e-14
+        if mode == 0:
+            assert (out == 7.0).all()
+        elif mode == 1:
+            assert rel(out[:, :q].cpu().numpy(), dots.cpu().numpy()) <= 1e-12
+            assert (out[:, q:] == 7.0).all()
+        else:
+            assert rel(out[:, 0].real.cpu().numpy(), nrm.cpu().numpy()) <= 1e-13
+            assert (out[:, 0].imag == 0).all() and (out[:, 1:] == 7.0).all()
+
+
 @pytest.mark.parametrize("precond,shifts", [
     ("line", [complex(0.5, 1.0), complex(0.0, 3.0), complex(2.0, 0.0)]),
     # block-Jacobi only copes when the shift dominates the radial coupling of thin cells
