@@ -230,7 +230,9 @@ def freq_response(torch, superlu: bool, shifts: int = 32):
 def profile_launches(torch, K):
     """Kernel launches for hardware-counter passes (rocprofv3 --pmc): every dispatch is a plain
     launch on torch's current stream.  A device-to-device copy of 512 MiB comes first; its
-    FETCH_SIZE / WRITE_SIZE give the calibration of the byte counters."""
+    FETCH_SIZE / WRITE_SIZE give the calibration of the byte counters.  Then the face kernels
+    (batched residual at 257x129; residual, JVP, VJP and the complex step at 1025x513) and the
+    radial-line preconditioner solve of batched GMRES (32 shifts, 1025x513)."""
     src = torch.empty(64 * 2 ** 20, dtype=torch.float64, device="cuda").fill_(1.0)
     dst = torch.empty_like(src)
     for _ in range(3):
@@ -260,6 +262,15 @@ def profile_launches(torch, K):
         K.jvp_odes(p, 0.5, g, Xd, Vd, Ud, dU=dU, dR=out)
         K.vjp_odes(p, 0.5, g, Xd, Vd, Ud, w)
         K.cs_jvp_odes(p, 0.5, g, Xd, Vd, Ud, dU=dU, dR=out)
+    # radial-line preconditioner of batched GMRES: factorisation and solves for 32 shifts
+    from pyeulerale_amd.gmres import BatchedGMRES
+    JU = K.cs_linearize_odes(p, 0.5, g, Xd, Vd, Ud, want=("states",))[0]
+    gm = BatchedGMRES(g, JU, 1j * np.logspace(1, 2, 32), restart=2)
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    rr = torch.randn((32, *g.cell_shape), dtype=torch.complex128, device="cuda", generator=gen)
+    zz = torch.empty_like(rr)
+    for _ in range(5):
+        gm.psolve(rr, zz)
     torch.cuda.synchronize()
 
 

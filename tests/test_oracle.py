@@ -225,3 +225,29 @@ def test_bsr_export_matches_dense(port):
     d = crandn(rng, U.shape)
     y = dense @ d.reshape(-1, order="F")
     assert rel(y, port.apply_odes_wrt_states_fwd(JU, d).reshape(-1, order="F")) <= 1e-13
+
+
+def test_directional_complex_step_kink_at_the_wake_cut(port):
+    """On an unpitched symmetric grid at zero incidence, rho w is exactly 0 and the radial faces of
+    the trailing-edge wake cut lie on the x axis, so the normal velocity q = n.j/rho of those
+    faces is exactly 0: a kink of |q| (f90:146).  A complex step along one input component keeps
+    Re q exactly 0 there, but one along a direction in every component gives Re q of order h^2
+    and either sign, so the directional complex step differs from the per-column one (the
+    reference's linearize), whatever h, and only in the cells beside the cut.  Pitching the grid
+    removes the exact zero and the two agree.  The directional comparator of the benchmark at
+    its unpitched 1025x513 grid shows this difference."""
+    rng = np.random.default_rng(1)
+    for alpha, expect_kink in ((0.0, True), (0.01, False)):
+        X, V, U = case(65, 33, thickness=0.04, mach=0.5, aoa=0.0, alpha=alpha)
+        prm = Params(aoa=0.0)
+        d = rng.standard_normal(U.shape)
+        J = port.compute_odes_wrt_states(prm, 0.5, X, V, U)
+        blk = np.real(port.apply_odes_wrt_states_fwd(J, d))
+        for h in (1e-12, 1e-30):
+            cs = port.cs_jvp(prm, 0.5, X, V, U, dU=d, h=h)
+            diff = np.abs(cs - blk).max(axis=0)                       # (M, N)
+            bad = set(np.argwhere(diff > 1e-10 * np.abs(blk).max())[:, 1].tolist())
+            if expect_kink:
+                assert bad and bad <= {0, U.shape[2] - 1}, bad
+            else:
+                assert not bad and rel(cs, blk) <= 1e-13
