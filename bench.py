@@ -227,6 +227,45 @@ def freq_response(torch, superlu: bool, shifts: int = 32):
     return res
 
 
+def residual_beyond_cache(torch, K, p, mach, g, Xd, Vd, Ud, Rd, alpha, alpha_dot, reps=20):
+    """The batched residual where consecutive launches cannot be served by the 256 MB Infinity
+    Cache: (a) 256 states per launch (537 MB of states and residuals), (b) the 64-state launch
+    with a 1 GiB buffer written between launches, outside the timed events.  Per-launch HIP
+    events; states per second of kernel time against the 64-state launch without flushing."""
+    def timed(fn, flush=None):
+        evs = []
+        for _ in range(reps):
+            if flush is not None:
+                flush()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            fn()
+            b.record()
+            evs.append((a, b))
+        torch.cuda.synchronize()
+        return float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    big = 256
+    _, _, U, M, N = make_case(257, 129, 0.04, mach, 0.0, alpha, alpha_dot, 20.0, big, seed=1)
+    Ub = torch.as_tensor(np.ascontiguousarray(np.swapaxes(U, -1, -2))).cuda()
+    Rb = torch.empty_like(Ub)
+    fn64 = lambda: K.compute_odes_batch(p, mach, g, Xd, Vd, Ud, Rd)
+    fn256 = lambda: K.compute_odes_batch(p, mach, g, Xd, Vd, Ub, Rb)
+    junk = torch.empty(2 ** 27, dtype=torch.float64, device="cuda")
+    flush = lambda: junk.fill_(1.0)
+    for _ in range(3):
+        fn64(), fn256()
+    ms64 = timed(fn64)
+    ms64f = timed(fn64, flush)
+    ms256 = timed(fn256)
+    b64 = Ud.shape[0]
+    return {"grid": "257x129", "states_per_launch": [b64, big],
+            "ms_per_launch_64": ms64, "ms_per_launch_64_cache_flushed": ms64f,
+            "ms_per_launch_256": ms256,
+            "states_per_s_64": b64 / (ms64 * 1e-3),
+            "states_per_s_64_cache_flushed": b64 / (ms64f * 1e-3),
+            "states_per_s_256": big / (ms256 * 1e-3)}
+
+
 def profile_launches(torch, K):
     """Kernel launches for hardware-counter passes (rocprofv3 --pmc): every dispatch is a plain
     launch on torch's current stream.  A device-to-device copy of 512 MiB comes first; its
@@ -540,7 +579,10 @@ def main():
     value = BATCH * args.steps / wall
 
     progress(f"residual batch: {kern_ms:.4f} ms per launch")
-    extra = {"products_257x129": products(torch, K, 257, 129, 0.04, mach, aoa, alpha,
+    cache = residual_beyond_cache(torch, K, p, mach, g, Xd, Vd, Ud, Rd, alpha, alpha_dot)
+    progress("residual beyond the Infinity Cache done")
+    extra = {"residual_beyond_infinity_cache": cache,
+             "products_257x129": products(torch, K, 257, 129, 0.04, mach, aoa, alpha,
                                           alpha_dot, 20.0, 50)}
     progress("products 257x129 done")
     if not args.no_large:
