@@ -95,13 +95,18 @@ __device__ B4 binv(const B4& x) {
                 best = zabs2(m[r][c]);
                 p = r;
             }
-        if (p != c)
+        // swap rows c and p by selects over the candidate rows: a row index known only at run
+        // time would put m in scratch memory
+#pragma unroll
+        for (int r = c + 1; r < 4; ++r) {
+            const bool sw = p == r;
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
-                Z t = m[c][j];
-                m[c][j] = m[p][j];
-                m[p][j] = t;
+                const Z a = m[c][j], b = m[r][j];
+                m[c][j] = sw ? b : a;
+                m[r][j] = sw ? a : b;
             }
+        }
         const Z piv = zinv(m[c][c]);
 #pragma unroll
         for (int j = 0; j < 8; ++j) m[c][j] = zmul(m[c][j], piv);
@@ -159,6 +164,60 @@ __device__ __forceinline__ long u2idx(int M, int N, int S, int m, int ij, int n)
     const long g = n / kLineGroup, nl = n % kLineGroup;
     return base + (((long)g * M + m) * 16 + ij) * kLineGroup + nl;
 }
+// after U2 (in doubles): the system's line blocks (m, m + off), off = -2..2, without the shift,
+// lines fastest: SB[((off + 2)*16 + 4i + j)*M*N + m*N + n], written by k_line_blocks for the
+// factorisation
+__host__ __device__ __forceinline__ long sb_offset(int M, int N, int S) {
+    return 2L * S * line_groups(N) * M * 4 * 16 * kLineGroup + line_groups(N) * M * 16L * kLineGroup;
+}
+__device__ __forceinline__ B4 sb_block(const double* SB, long C, int N, int n, int m, int off,
+                                       Z shift) {
+    B4 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            r.a[i][j] = Z{SB[(long)((off + 2) * 16 + 4 * i + j) * C + (long)m * N + n], 0.0};
+            if (off == 0 && i == j) r.a[i][j] = zsub(r.a[i][j], shift);
+        }
+    return r;
+}
+
+// The system's radial-line blocks in the line-major layout of sb_block, zero where m + off is
+// off the line: a tiled transpose of the blocks' (n, m) planes (m fastest) into (m, n) planes,
+// both sides coalesced.  The transposed system's block (m, m + off) is block (m + off, m) of J
+// transposed.  Grid (ceil(M/kTrM), ceil(N/kTrN), 80 planes).
+constexpr int kTrM = 16, kTrN = 64;
+
+__global__ void __launch_bounds__(256)
+k_line_blocks(int M, int N, const double* __restrict__ JU, int tr, double* __restrict__ SB) {
+    __shared__ double t[kTrN][kTrM + 1];
+    const int plane = blockIdx.z;
+    const int off = plane / 16 - 2, ij = plane % 16, i = ij / 4, j = ij % 4;
+    const int m0 = blockIdx.x * kTrM, n0 = blockIdx.y * kTrN;
+    const long C = (long)M * N;
+    int slot, a = i, b = j, dm = 0;
+    if (!tr) {
+        slot = off == -2 ? 6 : off == -1 ? 5 : off == 0 ? 0 : off == 1 ? 7 : 8;
+    } else {
+        slot = off == -2 ? 8 : off == -1 ? 7 : off == 0 ? 0 : off == 1 ? 5 : 6;
+        a = j;
+        b = i;
+        dm = off;
+    }
+    for (int k = threadIdx.x; k < kTrM * kTrN; k += 256) {
+        const int ml = k % kTrM, nl = k / kTrM, m = m0 + ml, n = n0 + nl;
+        double v = 0.0;
+        if (m < M && n < N && m + off >= 0 && m + off < M)
+            v = JU[((long)(slot * 4 + a) * 4 + b) * C + (long)n * M + m + dm];
+        t[nl][ml] = v;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kTrM * kTrN; k += 256) {
+        const int nl = k % kTrN, ml = k / kTrN, m = m0 + ml, n = n0 + nl;
+        if (m < M && n < N) SB[(long)plane * C + (long)m * N + n] = t[nl][ml];
+    }
+}
 __device__ __forceinline__ void store_b(double* F, int M, int N, int sh, int m, int slot, int n,
                                         const B4& b) {
 #pragma unroll
@@ -183,52 +242,39 @@ __device__ __forceinline__ B4 load_b(const double* F, int M, int N, int sh, int 
     return b;
 }
 
-// Radial neighbour blocks of row m on the line: ii (m-2), i (m-1), c, o (m+1), oo (m+2).
-// Transposed system: block (m, m+k) of J^T is block (m+k, m) of J transposed, i.e. the block
-// of cell m+k whose neighbour slot points back at m.
-__device__ __forceinline__ B4 line_block(const double* JU, int M, int N, int n, int m, int off,
-                                         bool tr, Z shift) {
-    const int C = M * N;
-    // slots: ii = 6, i = 5, c = 0, o = 7, oo = 8 ; reverse slot of an offset
-    if (!tr) {
-        const int s = off == -2 ? 6 : off == -1 ? 5 : off == 0 ? 0 : off == 1 ? 7 : 8;
-        return load_jblock(JU, C, n * M + m, s, false, off == 0 ? shift : Z{0.0, 0.0});
-    }
-    const int s = off == -2 ? 8 : off == -1 ? 7 : off == 0 ? 0 : off == 1 ? 5 : 6;
-    return load_jblock(JU, C, n * M + m + off, s, true, off == 0 ? shift : Z{0.0, 0.0});
-}
-
 __global__ void __launch_bounds__(64)
-k_line_factor(int M, int N, int S, const double* __restrict__ JU, const double* __restrict__ sh,
-              int tr, double* __restrict__ F) {
+k_line_factor(int M, int N, int S, const double* __restrict__ sh, int tr, double* __restrict__ F) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = blockIdx.y;
     if (n >= N || s >= S) return;
+    const long C = (long)M * N;
+    const double* SB = F + sb_offset(M, N, S);
     // adjoint: (J - s I)^H = J^T - conj(s) I
     const Z shift = tr ? Z{sh[2 * s], -sh[2 * s + 1]} : Z{sh[2 * s], sh[2 * s + 1]};
+    const Z zero = {0.0, 0.0};
     for (int m = 0; m < M; ++m) {
-        B4 D = line_block(JU, M, N, n, m, 0, tr, shift);
+        B4 D = sb_block(SB, C, N, n, m, 0, shift);
         B4 L2, L1;
         if (m >= 2) {
-            L2 = bmul(line_block(JU, M, N, n, m, -2, tr, shift), load_b(F, M, N, s, m - 2, 2, n));
+            L2 = bmul(sb_block(SB, C, N, n, m, -2, zero), load_b(F, M, N, s, m - 2, 2, n));
             // D -= L2 * U2(m-2),  U2(m-2) = block (m-2, m)
-            D = bsub(D, bmul(L2, line_block(JU, M, N, n, m - 2, 2, tr, shift)));
+            D = bsub(D, bmul(L2, sb_block(SB, C, N, n, m - 2, 2, zero)));
             store_b(F, M, N, s, m, 0, n, L2);
         }
         if (m >= 1) {
-            B4 B1 = line_block(JU, M, N, n, m, -1, tr, shift);
+            B4 B1 = sb_block(SB, C, N, n, m, -1, zero);
             if (m >= 2) B1 = bsub(B1, bmul(L2, load_b(F, M, N, s, m - 2, 3, n)));
             L1 = bmul(B1, load_b(F, M, N, s, m - 1, 2, n));
             D = bsub(D, bmul(L1, load_b(F, M, N, s, m - 1, 3, n)));
             store_b(F, M, N, s, m, 1, n, L1);
         }
         if (m + 1 < M) {
-            B4 U1 = line_block(JU, M, N, n, m, 1, tr, shift);
-            if (m >= 1) U1 = bsub(U1, bmul(L1, line_block(JU, M, N, n, m - 1, 2, tr, shift)));
+            B4 U1 = sb_block(SB, C, N, n, m, 1, zero);
+            if (m >= 1) U1 = bsub(U1, bmul(L1, sb_block(SB, C, N, n, m - 1, 2, zero)));
             store_b(F, M, N, s, m, 3, n, U1);
         }
         if (s == 0 && m + 2 < M) {
-            const B4 U2 = line_block(JU, M, N, n, m, 2, tr, shift);
+            const B4 U2 = sb_block(SB, C, N, n, m, 2, zero);
 #pragma unroll
             for (int ij = 0; ij < 16; ++ij) F[u2idx(M, N, S, m, ij, n)] = U2.a[ij / 4][ij % 4].r;
         }
@@ -914,14 +960,17 @@ extern "C" {
 const char* pea_resolvent_last_error(void) { return g_rerr; }
 
 long long pea_line_factor_bytes(int M, int N, int S) {
-    return (long long)line_groups(N) * kLineGroup * M * 16 * (S * 4 * 16 + 8);
+    return (long long)line_groups(N) * kLineGroup * M * 16 * (S * 4 * 16 + 8) +
+           80LL * M * N * 8;
 }
 
 int pea_line_factor(int M, int N, int S, const double* JU, const double* shifts, int transposed,
                     double* F, void* stream) {
     if (M < 4 || N < 5 || S < 1 || S > 65535) return rfail(PEA_ERR_DIMS, "bad dimensions");
     if (!JU || !shifts || !F) return rfail(PEA_ERR_NULL, "null argument");
-    k_line_factor<<<dim3((N + 63) / 64, S), 64, 0, (hipStream_t)stream>>>(M, N, S, JU, shifts,
+    k_line_blocks<<<dim3((M + kTrM - 1) / kTrM, (N + kTrN - 1) / kTrN, 80), 256, 0,
+                    (hipStream_t)stream>>>(M, N, JU, transposed, F + sb_offset(M, N, S));
+    k_line_factor<<<dim3((N + 63) / 64, S), 64, 0, (hipStream_t)stream>>>(M, N, S, shifts,
                                                                          transposed, F);
     return hipGetLastError() == hipSuccess ? PEA_OK : rfail(PEA_ERR_HIP, "k_line_factor");
 }
