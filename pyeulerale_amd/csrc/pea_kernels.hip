@@ -105,16 +105,42 @@ PEA_HD int vtx_index(const Grid& g, int m, int n, int vtx) {
 // =============================================================================== loaders
 // A loader turns device arrays into scalars of type T: states, vertices, velocities, Mach.
 
+// Loads through a buffer descriptor: the element offset is one 32-bit lane value shared by
+// every plane of an array, and the plane is a scalar offset, so no 64-bit address is formed per
+// load.  The descriptor covers `bytes` (offsets beyond read 0); word 3 is the gfx9 raw-buffer
+// format.  Arrays up to 4 GiB.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const double* p, long bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ double buf_load(__amdgpu_buffer_rsrc_t r, int elem, int plane_elems) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, elem * 8,
+                                                                          plane_elems * 8, 0));
+}
+// complex element (re, im) of an interleaved complex array
+__device__ __forceinline__ double2 buf_load_cx(__amdgpu_buffer_rsrc_t r, int elem, int plane_elems) {
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, elem * 16,
+                                                                            plane_elems * 16, 0));
+}
+
 struct LdReal {
     const double *X, *V, *U;
     double mach;
     Grid g;
-    PEA_HD void state(int c, double* u) const {
+    __device__ void state(int c, double* u) const {
+        const auto r = buf_rsrc(U, 32L * g.C);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) u[v] = U[v * g.C + c];
+        for (int v = 0; v < 4; ++v) u[v] = buf_load(r, c, v * g.C);
     }
-    PEA_HD void vertex(int p, double* x) const { x[0] = X[p]; x[1] = X[g.P + p]; }
-    PEA_HD void velocity(int p, double* w) const { w[0] = V[p]; w[1] = V[g.P + p]; }
+    __device__ void vertex(int p, double* x) const {
+        const auto r = buf_rsrc(X, 16L * g.P);
+        x[0] = buf_load(r, p, 0);
+        x[1] = buf_load(r, p, g.P);
+    }
+    __device__ void velocity(int p, double* w) const {
+        const auto r = buf_rsrc(V, 16L * g.P);
+        w[0] = buf_load(r, p, 0);
+        w[1] = buf_load(r, p, g.P);
+    }
     PEA_HD double machv() const { return mach; }
 };
 
@@ -160,21 +186,35 @@ struct LdDual {
 // Tangents on the states only: geometry and Mach stay plain FP64.
 template <int K>
 struct LdDualU {
+    static_assert(K % 2 == 0, "tangent lanes come in (re, im) pairs of complex directions");
     const double *X, *V, *U;
     const double* dU;
     double mach;
     Grid g;
-    PEA_HD void state(int c, Dual<K>* u) const {
+    __device__ void state(int c, Dual<K>* u) const {
+        const auto r = buf_rsrc(U, 32L * g.C);
+        const auto rd = buf_rsrc(dU, 64L * g.C * (K / 2));
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int e = v * g.C + c;
-            u[v].v = U[e];
+            u[v].v = buf_load(r, c, v * g.C);
 #pragma unroll
-            for (int j = 0; j < K; ++j) u[v].d[j] = dU[2 * ((long)(j >> 1) * 4 * g.C + e) + (j & 1)];
+            for (int d = 0; d < K / 2; ++d) {
+                const double2 z = buf_load_cx(rd, c, d * 4 * g.C + v * g.C);
+                u[v].d[2 * d] = z.x;
+                u[v].d[2 * d + 1] = z.y;
+            }
         }
     }
-    PEA_HD void vertex(int p, double* x) const { x[0] = X[p]; x[1] = X[g.P + p]; }
-    PEA_HD void velocity(int p, double* w) const { w[0] = V[p]; w[1] = V[g.P + p]; }
+    __device__ void vertex(int p, double* x) const {
+        const auto r = buf_rsrc(X, 16L * g.P);
+        x[0] = buf_load(r, p, 0);
+        x[1] = buf_load(r, p, g.P);
+    }
+    __device__ void velocity(int p, double* w) const {
+        const auto r = buf_rsrc(V, 16L * g.P);
+        w[0] = buf_load(r, p, 0);
+        w[1] = buf_load(r, p, g.P);
+    }
     PEA_HD double machv() const { return mach; }
 };
 
@@ -185,15 +225,25 @@ struct LdCxU {
     double mach, h;
     int part;
     Grid g;
-    PEA_HD void state(int c, Cx* u) const {
+    __device__ void state(int c, Cx* u) const {
+        const auto r = buf_rsrc(U, 32L * g.C);
+        const auto rd = buf_rsrc(dU, 64L * g.C);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int e = v * g.C + c;
-            u[v] = Cx{U[e], h * dU[2 * e + part]};
+            const double2 z = buf_load_cx(rd, c, v * g.C);
+            u[v] = Cx{buf_load(r, c, v * g.C), h * (part ? z.y : z.x)};
         }
     }
-    PEA_HD void vertex(int p, double* x) const { x[0] = X[p]; x[1] = X[g.P + p]; }
-    PEA_HD void velocity(int p, double* w) const { w[0] = V[p]; w[1] = V[g.P + p]; }
+    __device__ void vertex(int p, double* x) const {
+        const auto r = buf_rsrc(X, 16L * g.P);
+        x[0] = buf_load(r, p, 0);
+        x[1] = buf_load(r, p, g.P);
+    }
+    __device__ void velocity(int p, double* w) const {
+        const auto r = buf_rsrc(V, 16L * g.P);
+        w[0] = buf_load(r, p, 0);
+        w[1] = buf_load(r, p, g.P);
+    }
     PEA_HD double machv() const { return mach; }
 };
 
