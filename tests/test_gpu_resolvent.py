@@ -48,14 +48,32 @@ def test_block_multi_matches_single_products(sd):
             assert rel(y[k].cpu().numpy(), ref.cpu().numpy()) <= 1e-13
 
 
+@pytest.fixture(scope="module")
+def sd_ragged():
+    """A grid whose line count (N = 42) is not a multiple of the solve's 16-line groups and whose
+    line length (M = 21) is not a multiple of its 8-step blocks."""
+    from pyeulerale_amd.core import SpatialDiscretization
+    X, V, U = case(43, 22, thickness=0.12, mach=0.5, aoa=1.25, alpha=0.01, alpha_dot=0.004,
+                   noise=1e-3)
+    s = SpatialDiscretization(vertices=X, angle_of_attack=1.25, mach_number=0.5)
+    s.velocities[:] = torch.as_tensor(V).cuda()
+    s.states[:] = torch.as_tensor(U).cuda()
+    s.linearize()
+    print("[resolvent] 43x22 linearized", flush=True)
+    return s
+
+
+@pytest.mark.parametrize("grid", ["33x17", "43x22"])
 @pytest.mark.parametrize("adjoint", [False, True])
-def test_line_preconditioner_solves_each_radial_line(sd, adjoint):
+def test_line_preconditioner_solves_each_radial_line(request, grid, adjoint):
     """pea_line_factor/pea_line_solve against dense solves of every radial line's system in numpy:
     the line's blocks ii, i, c, o, oo of dR/dU minus s I (transposed, minus conj(s) I, for the
     adjoint), for several shifts at once."""
     from pyeulerale_amd import _lib
     from pyeulerale_amd import kernels as K
     from pyeulerale_amd.gmres import _p
+    sd = request.getfixturevalue("sd" if grid == "33x17" else "sd_ragged")
+    print(f"[resolvent] line preconditioner {grid} adjoint={adjoint}", flush=True)
     g = sd._grid
     M, N = g.M, g.N
     JU = sd._get_blocks()["U"]
