@@ -274,6 +274,17 @@ int pea_gs_update(int S, int n, int q, const double *V, long long ldv, long long
                   const double *h, long long bsh, double *w, long long bsw, int mode, int nb,
                   double *P, double *out, long long bso, void *stream);
 
+/* One Gram-Schmidt step of batched GMRES with selective reorthogonalisation (Daniel, Gragg,
+ * Kaufman, Stewart 1976): h1 = V^H w and |w|^2 in one pass over the q basis rows; w -= V h1;
+ * only for shifts where |h1|^2 > |w|^2/2 the projection is repeated (h2 = V^H w, w -= V h2).
+ * Outputs h[s*bsh + i] = h1 + h2 (complex, i < q) and nrm2[s] = |w_new|^2 (double); flags[s] = 1
+ * where the projection was repeated.  Workspace: P of S*nb*(q+1) complex values, hw of
+ * 2*S*(q+1) complex values.  1 <= q <= 32; V and w 16-byte aligned.  Strides as pea_gs_update.
+ * Part of the batched GMRES; the reference solves each shift directly (core.py:562, 596). */
+int pea_gs_orthogonalize(int S, int n, int q, const double *V, long long ldv, long long bsv,
+                         double *w, long long bsw, int nb, double *P, double *hw, int *flags,
+                         double *h, long long bsh, double *nrm2, void *stream);
+
 /* Block-Jacobi preconditioner: inverses of the 4x4 diagonal blocks (J_cc - s I);
  * F holds S*16*M*N complex values. */
 int pea_jacobi_factor(int M, int N, int S, const double *JU, const double *shifts,

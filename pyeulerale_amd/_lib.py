@@ -76,6 +76,8 @@ _SIGS = {
                      _P, _c.c_longlong, _P],
     "pea_gs_update": [_I, _I, _I, _P, _c.c_longlong, _c.c_longlong, _P, _c.c_longlong, _P,
                       _c.c_longlong, _I, _I, _P, _P, _c.c_longlong, _P],
+    "pea_gs_orthogonalize": [_I, _I, _I, _P, _c.c_longlong, _c.c_longlong, _P, _c.c_longlong,
+                             _I, _P, _P, _P, _P, _c.c_longlong, _P, _P],
     "pea_dense_lu_solve": [_I, _I, _I, _P, _P, _P, _P],
 }
 _RESTYPES = {"pea_last_error": _c.c_char_p, "pea_vjp_workspace_bytes": _c.c_longlong,
@@ -85,7 +87,7 @@ _RESTYPES = {"pea_last_error": _c.c_char_p, "pea_vjp_workspace_bytes": _c.c_long
 _RESOLVENT = {"pea_block_multi", "pea_line_factor", "pea_line_solve", "pea_jacobi_factor",
               "pea_jacobi_solve", "pea_ring_band", "pea_ring_couple", "pea_dense_lu_factor",
               "pea_dense_lu_solve", "pea_gmres_rotate",
-              "pea_multidot", "pea_gs_update"}
+              "pea_multidot", "pea_gs_update", "pea_gs_orthogonalize"}
 
 _lib = None
 

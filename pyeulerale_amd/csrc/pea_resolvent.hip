@@ -775,17 +775,19 @@ __device__ __forceinline__ void block_sums(const Z* acc, int nout, double (*s_re
 // Stage 1: workgroup (chunk b, shift s) reads each element of w once and the q <= QMAX rows of V
 // at it, and writes the chunk's q sums to P[((s*nb + b)*pq + i)] (complex; pq >= q is the row
 // count of the whole projection when rows come in slices, P then offset to the slice).
-template <int QMAX>
+// NORM: also |w|^2, written after the q sums (pq >= q + 1).
+template <int QMAX, bool NORM = false>
 __global__ void __launch_bounds__(kDotThreads)
 k_multidot_partial(int n, int q, const double* __restrict__ V, long ldv, long bsv,
                    const double* __restrict__ w, long bsw, int nb, int pq, double* __restrict__ P) {
     __shared__ double s_red[kDotThreads / 64][2 * QMAX];
+    __shared__ double s_redn[kDotThreads / 64][2];
     const int b = blockIdx.x, s = blockIdx.y, tid = threadIdx.x;
     const long chunk = ((long)n + nb - 1) / nb;
     const long k0 = (long)b * chunk, k1 = min((long)n, k0 + chunk);
     const double2* Vs = reinterpret_cast<const double2*>(V) + (long)s * bsv;
     const double2* ws = reinterpret_cast<const double2*>(w) + (long)s * bsw;
-    Z acc[QMAX];
+    Z acc[QMAX], accn = {0.0, 0.0};
 #pragma unroll
     for (int i = 0; i < QMAX; ++i) acc[i] = Z{0.0, 0.0};
 #pragma unroll 1
@@ -798,8 +800,11 @@ k_multidot_partial(int n, int q, const double* __restrict__ V, long ldv, long bs
                 acc[i].r = fma(v.x, x.x, fma(v.y, x.y, acc[i].r));
                 acc[i].i = fma(v.x, x.y, fma(-v.y, x.x, acc[i].i));
             }
+        if (NORM) accn.r = fma(x.x, x.x, fma(x.y, x.y, accn.r));
     }
-    block_sums<QMAX>(acc, q, s_red, P + 2 * ((long)s * nb + b) * pq);
+    double* out = P + 2 * ((long)s * nb + b) * pq;
+    block_sums<QMAX>(acc, q, s_red, out);
+    if (NORM) block_sums<1>(&accn, 1, s_redn, out + 2 * q);
 }
 
 // Stage 2: one wave per output (s, i).  Lane l adds chunks l, l + 64, ...; a butterfly over the
@@ -836,10 +841,11 @@ template <int QMAX>
 __global__ void __launch_bounds__(kDotThreads)
 k_gs_update(int n, int q, const double* __restrict__ V, long ldv, long bsv,
             const double* __restrict__ h, long bsh, double* __restrict__ w, long bsw, int nb,
-            int mode, double* __restrict__ P) {
+            int mode, double* __restrict__ P, const int* __restrict__ only = nullptr) {
     __shared__ double s_h[2 * QMAX];
     __shared__ double s_red[kDotThreads / 64][2 * QMAX];
     const int b = blockIdx.x, s = blockIdx.y, tid = threadIdx.x;
+    if (only && !only[s]) return;   // whole workgroups: shifts the caller leaves out
     if (tid < 2 * q) s_h[tid] = h[2 * (long)s * bsh + tid];
     __syncthreads();
     const long chunk = ((long)n + nb - 1) / nb;
@@ -887,9 +893,113 @@ k_gs_update(int n, int q, const double* __restrict__ V, long ldv, long bsv,
 
 template <int QMAX>
 void launch_gs_update(int S, int n, int q, const double* V, long ldv, long bsv, const double* h,
-                      long bsh, double* w, long bsw, int nb, int mode, double* P, hipStream_t st) {
+                      long bsh, double* w, long bsw, int nb, int mode, double* P, hipStream_t st,
+                      const int* only = nullptr) {
     k_gs_update<QMAX><<<dim3(nb, S), kDotThreads, 0, st>>>(n, q, V, ldv, bsv, h, bsh, w, bsw, nb,
-                                                           mode, P);
+                                                           mode, P, only);
+}
+
+// Selective reorthogonalisation (Daniel, Gragg, Kaufman and Stewart, Math. Comp. 30 (1976)):
+// after w -= V h1, repeat the projection only if it removed more than half of |w|^2, i.e.
+// |h1|^2 > |w|^2 / 2.  First update pass: hA holds h1 (q values) and |w|^2 (index q), stride
+// q + 1.  Every workgroup evaluates the test the same way; the shift's flag goes to flags[s].
+// The pass writes q + 1 partial sums per (chunk, s): the next projections V^H w_new if the
+// shift is flagged (zeros otherwise), and |w_new|^2.
+template <int QMAX>
+__global__ void __launch_bounds__(kDotThreads)
+k_dgks_update(int n, int q, const double* __restrict__ V, long ldv, long bsv,
+              const double* __restrict__ hA, double* __restrict__ w, long bsw, int nb,
+              double* __restrict__ P, int* __restrict__ flags) {
+    __shared__ double s_h[2 * QMAX];
+    __shared__ double s_red[kDotThreads / 64][2 * QMAX];
+    __shared__ double s_redn[kDotThreads / 64][2];
+    __shared__ int s_flag;
+    const int b = blockIdx.x, s = blockIdx.y, tid = threadIdx.x;
+    const double* hs = hA + 2 * (long)s * (q + 1);
+    if (tid < 2 * q) s_h[tid] = hs[tid];
+    if (tid == 0) {
+        double h2 = 0.0;
+        for (int i = 0; i < q; ++i) h2 += hs[2 * i] * hs[2 * i] + hs[2 * i + 1] * hs[2 * i + 1];
+        s_flag = h2 > 0.5 * hs[2 * q];
+        if (b == 0) flags[s] = s_flag;
+    }
+    __syncthreads();
+    const bool reorth = s_flag;
+    const long chunk = ((long)n + nb - 1) / nb;
+    const long k0 = (long)b * chunk, k1 = min((long)n, k0 + chunk);
+    const double2* Vs = reinterpret_cast<const double2*>(V) + (long)s * bsv;
+    double2* ws = reinterpret_cast<double2*>(w) + (long)s * bsw;
+    Z acc[QMAX], accn = {0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < QMAX; ++i) acc[i] = Z{0.0, 0.0};
+#pragma unroll 1
+    for (long k = k0 + tid; k < k1; k += kDotThreads) {
+        const double2 x0 = ws[k];
+        Z x = {x0.x, x0.y};
+        Z v[QMAX];
+#pragma unroll
+        for (int i = 0; i < QMAX; ++i) {
+            if (i < q) {
+                const double2 t = Vs[(long)i * ldv + k];
+                v[i] = Z{t.x, t.y};
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < QMAX; ++i) {   // x -= v_i h_i, rows in order
+            if (i < q) {
+                const double hr = s_h[2 * i], hi = s_h[2 * i + 1];
+                x = Z{fma(-v[i].r, hr, fma(v[i].i, hi, x.r)), fma(-v[i].r, hi, fma(-v[i].i, hr, x.i))};
+            }
+        }
+        ws[k] = make_double2(x.r, x.i);
+        if (reorth) {
+#pragma unroll
+            for (int i = 0; i < QMAX; ++i)
+                if (i < q) {   // conj(v) * x
+                    acc[i].r = fma(v[i].r, x.r, fma(v[i].i, x.i, acc[i].r));
+                    acc[i].i = fma(v[i].r, x.i, fma(-v[i].i, x.r, acc[i].i));
+                }
+        }
+        accn.r = fma(x.r, x.r, fma(x.i, x.i, accn.r));
+    }
+    double* out = P + 2 * ((long)s * nb + b) * (q + 1);
+    block_sums<QMAX>(acc, q, s_red, out);
+    block_sums<1>(&accn, 1, s_redn, out + 2 * q);
+}
+
+// Last stage, one wave per shift: h = h1 + h2 (h2 only where flagged) and |w|^2, from the
+// second update's partial norms where flagged, else from the first update's (hB index q).
+__global__ void __launch_bounds__(64)
+k_dgks_finish(int S, int q, int nb, const double* __restrict__ P, const int* __restrict__ flags,
+              const double* __restrict__ hA, const double* __restrict__ hB, double* __restrict__ h,
+              long bsh, double* __restrict__ nrm2) {
+    const int s = blockIdx.x, lane = threadIdx.x;
+    if (s >= S) return;
+    const bool reorth = flags[s];
+    const double* a = hA + 2 * (long)s * (q + 1);
+    const double* c = hB + 2 * (long)s * (q + 1);
+    for (int i = lane; i < q; i += 64) {
+        h[2 * ((long)s * bsh + i)] = a[2 * i] + (reorth ? c[2 * i] : 0.0);
+        h[2 * ((long)s * bsh + i) + 1] = a[2 * i + 1] + (reorth ? c[2 * i + 1] : 0.0);
+    }
+    double t = 0.0;
+    if (reorth)
+        for (int b = lane; b < nb; b += 64) t += P[2 * ((long)s * nb + b)];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
+    if (lane == 0) nrm2[s] = reorth ? t : c[2 * q];
+}
+
+template <int QMAX>
+void dgks_passes(int S, int n, int q, const double* V, long ldv, long bsv, double* w, long bsw,
+                 int nb, double* P, double* hA, double* hB, int* flags, hipStream_t st) {
+    k_multidot_partial<QMAX, true><<<dim3(nb, S), kDotThreads, 0, st>>>(n, q, V, ldv, bsv, w, bsw,
+                                                                        nb, q + 1, P);
+    k_multidot_final<<<S * (q + 1), 64, 0, st>>>(S, q + 1, nb, P, hA, q + 1);
+    k_dgks_update<QMAX><<<dim3(nb, S), kDotThreads, 0, st>>>(n, q, V, ldv, bsv, hA, w, bsw, nb, P,
+                                                             flags);
+    k_multidot_final<<<S * (q + 1), 64, 0, st>>>(S, q + 1, nb, P, hB, q + 1);
+    launch_gs_update<QMAX>(S, n, q, V, ldv, bsv, hB, q + 1, w, bsw, nb, 2, P, st, flags);
 }
 constexpr int kGsMaxRows = 32;
 
@@ -1119,6 +1229,24 @@ int pea_gs_update(int S, int n, int q, const double* V, long long ldv, long long
         k_multidot_final<<<S * nout, 64, 0, st>>>(S, nout, nb, P, out, bso);
     }
     return hipGetLastError() == hipSuccess ? PEA_OK : rfail(PEA_ERR_HIP, "k_gs_update");
+}
+
+int pea_gs_orthogonalize(int S, int n, int q, const double* V, long long ldv, long long bsv,
+                         double* w, long long bsw, int nb, double* P, double* hw, int* flags,
+                         double* h, long long bsh, double* nrm2, void* stream) {
+    if (S < 1 || n < 1 || nb < 1 || S > 65535) return rfail(PEA_ERR_DIMS, "bad dimensions");
+    if (q < 1 || q > kGsMaxRows) return rfail(PEA_ERR_DIMS, "q must be 1..32");
+    if (!V || !w || !P || !hw || !flags || !h || !nrm2) return rfail(PEA_ERR_NULL, "null argument");
+    if (((uintptr_t)V | (uintptr_t)w) & 15) return rfail(PEA_ERR_DIMS, "V and w must be 16-byte aligned");
+    hipStream_t st = (hipStream_t)stream;
+    double* hA = hw;
+    double* hB = hw + 2L * S * (q + 1);
+    if (q <= 4) dgks_passes<4>(S, n, q, V, ldv, bsv, w, bsw, nb, P, hA, hB, flags, st);
+    else if (q <= 8) dgks_passes<8>(S, n, q, V, ldv, bsv, w, bsw, nb, P, hA, hB, flags, st);
+    else if (q <= 16) dgks_passes<16>(S, n, q, V, ldv, bsv, w, bsw, nb, P, hA, hB, flags, st);
+    else dgks_passes<kGsMaxRows>(S, n, q, V, ldv, bsv, w, bsw, nb, P, hA, hB, flags, st);
+    k_dgks_finish<<<S, 64, 0, st>>>(S, q, nb, P, flags, hA, hB, h, bsh, nrm2);
+    return hipGetLastError() == hipSuccess ? PEA_OK : rfail(PEA_ERR_HIP, "k_dgks");
 }
 
 }  // extern "C"

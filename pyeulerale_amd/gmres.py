@@ -8,7 +8,9 @@ The reference solves one shift at a time with SuperLU on a freshly assembled CSR
   tangent (adjoint for the transposed system) per vector, no assembled matrix;
 * right preconditioning per shift: ``"line"`` (block LU of every radial line,
   ``pea_line_factor``/``pea_line_solve``), ``"jacobi"`` (4x4 diagonal blocks) or ``None``;
-* orthogonalisation: classical Gram-Schmidt applied twice (CGS2) as batched matrix products;
+* orthogonalisation: classical Gram-Schmidt with selective reorthogonalisation (the projection is
+  repeated only where it removed more than half of the vector's norm squared), in two or three
+  fused passes over the basis;
 * the Hessenberg matrices stay on the device: ``pea_gmres_rotate`` applies the Givens rotations
   of each step, one thread per shift, and the restart's triangular solves are batched.  An
   Arnoldi step is thus device work only; the step for each index j is captured once as a CUDA
@@ -128,26 +130,36 @@ class BatchedGMRES:
 
     # ------------------------------------------------------------------ solve
     def _step(self, j, st):
-        """Arnoldi step j for all shifts, entirely on the device: w = A P^-1 v_j, CGS2 against
-        v_0..v_j, v_{j+1}, column j of H, and its Givens update (``pea_gmres_rotate``)."""
+        """Arnoldi step j for all shifts, entirely on the device: w = A P^-1 v_j, Gram-Schmidt
+        against v_0..v_j, v_{j+1}, column j of H, and its Givens update (``pea_gmres_rotate``)."""
         S, n, R = self.S, st["n"], self.restart
         V, w, z = st["V"], st["w"], st["z"]
         w.copy_(V[:, j].reshape(w.shape))
         self.psolve(w, z)
         self.apply(z, w)
         q = j + 1
-        # CGS2: two passes of h = V^H w, w -= V h, in three passes over the basis rows:
-        # h1 = V^H w (pea_multidot: K split into chunks, which a library GEMM of this shape,
-        # <= 31 x 1 output and K = 4C, does not do); w -= V h1 together with h2 = V^H w; and
-        # w -= V h2 together with |w|^2 (pea_gs_update).
-        h1, h2 = st["h1"], st["h2"]
-        self._dots(st, V, R + 1, q, w, h1)
-        self._gs_update(st, V, R + 1, q, h1, w, 1, h2)
-        self._gs_update(st, V, R + 1, q, h2, w, 2, st["nrm"])
-        hn = st["nrm"][:, 0].real.sqrt()
-        torch.div(w.reshape(S, n), hn.clamp_min(1e-300)[:, None], out=V[:, j + 1])
         H = st["H"]
-        torch.add(h1[:, :q], h2[:, :q], out=H[:, :q, j])
+        if q <= 32:
+            # classical Gram-Schmidt with selective reorthogonalisation (pea_gs_orthogonalize):
+            # two passes over the basis rows, a third only for shifts whose first projection
+            # removed more than half of |w|^2
+            h = st["h1"]
+            _lib.call("pea_gs_orthogonalize", S, n, q, _p(V), n, (R + 1) * n, _p(w), n,
+                      st["nb"], _p(st["P"]), _p(st["hw"]), _p(st["flags"]), _p(h), h.shape[1],
+                      _p(st["nrm2"]), K._stream())
+            hn = st["nrm2"].sqrt()
+            H[:, :q, j] = h[:, :q]
+        else:
+            # CGS2 in three passes over the basis rows: h1 = V^H w (pea_multidot: K split into
+            # chunks, which a library GEMM of this shape does not do); w -= V h1 together with
+            # h2 = V^H w; w -= V h2 together with |w|^2 (pea_gs_update)
+            h1, h2 = st["h1"], st["h2"]
+            self._dots(st, V, R + 1, q, w, h1)
+            self._gs_update(st, V, R + 1, q, h1, w, 1, h2)
+            self._gs_update(st, V, R + 1, q, h2, w, 2, st["nrm"])
+            hn = st["nrm"][:, 0].real.sqrt()
+            torch.add(h1[:, :q], h2[:, :q], out=H[:, :q, j])
+        torch.div(w.reshape(S, n), hn.clamp_min(1e-300)[:, None], out=V[:, j + 1])
         H[:, j + 1, j] = hn.to(C128)
         _lib.call("pea_gmres_rotate", S, R, j, _p(H), _p(st["cs"]), _p(st["sn"]), _p(st["g"]),
                   _p(st["est"]), K._stream())
@@ -204,6 +216,9 @@ class BatchedGMRES:
                   nrm=torch.zeros((S, 1), dtype=C128, device=dev),
                   nb=max(8, min(1024, 2048 // S)))
         st["P"] = torch.empty((S, st["nb"], R + 1), dtype=C128, device=dev)
+        st["hw"] = torch.empty((2, S, R + 1), dtype=C128, device=dev)
+        st["flags"] = torch.zeros(S, dtype=torch.int32, device=dev)
+        st["nrm2"] = torch.zeros(S, dtype=torch.float64, device=dev)
         w = st["w"]
         bnorm = torch.linalg.vector_norm(b.reshape(S, n), dim=1).cpu().numpy()
         bnorm[bnorm == 0] = 1.0

@@ -159,6 +159,55 @@ def test_gs_update_matches_torch(q):
             assert (out[:, 0].imag == 0).all() and (out[:, 1:] == 7.0).all()
 
 
+@pytest.mark.parametrize("q", [3, 13, 32])
+def test_gs_orthogonalize_selective(q):
+    """pea_gs_orthogonalize against the same steps in torch: h1 = V^H w, w -= V h1, and, only for
+    shifts where |h1|^2 > |w|^2/2, h2 = V^H w, w -= V h2; outputs h1 + h2 and |w|^2.  Basis rows
+    orthonormal per shift; two shifts with w mostly in their span (projection repeated), two
+    with random w (not repeated)."""
+    from pyeulerale_amd import _lib
+    from pyeulerale_amd import kernels as K
+    from pyeulerale_amd._lib import PeaError
+    from pyeulerale_amd.gmres import _p
+    print(f"[resolvent] gs_orthogonalize q={q}", flush=True)
+    c128 = torch.complex128
+    S, n, rows, nb = 4, 5003, 33, 9
+    rng = np.random.default_rng(q)
+    Q = [np.linalg.qr(crandn(rng, (n, rows)))[0].T for _ in range(S)]     # orthonormal rows
+    V = torch.as_tensor(np.stack(Q)).cuda().contiguous()
+    w0 = np.empty((S, n), dtype=complex)
+    for k in range(S):
+        if k < 2:
+            w0[k] = crandn(rng, q) @ Q[k][:q] + 1e-3 * crandn(rng, n)
+        else:
+            w0[k] = crandn(rng, n)
+    wd = torch.as_tensor(w0).cuda()
+    P = torch.empty((S, nb, rows + 1), dtype=c128, device="cuda")
+    hw = torch.empty((2, S, rows + 1), dtype=c128, device="cuda")
+    flags = torch.full((S,), -1, dtype=torch.int32, device="cuda")
+    h = torch.full((S, rows + 2), 7.0, dtype=c128, device="cuda")
+    nrm2 = torch.zeros(S, dtype=torch.float64, device="cuda")
+    _lib.call("pea_gs_orthogonalize", S, n, q, _p(V), n, rows * n, _p(wd), n, nb, _p(P), _p(hw),
+              _p(flags), _p(h), rows + 2, _p(nrm2), K._stream())
+    Vq = np.stack(Q)[:, :q]
+    for k in range(S):
+        h1 = Vq[k].conj() @ w0[k]
+        w1 = w0[k] - Vq[k].T @ h1
+        reorth = np.vdot(h1, h1).real > 0.5 * np.vdot(w0[k], w0[k]).real
+        assert int(flags[k]) == int(reorth) == int(k < 2)
+        if reorth:
+            h2 = Vq[k].conj() @ w1
+            w1 = w1 - Vq[k].T @ h2
+            h1 = h1 + h2
+        assert rel(wd[k].cpu().numpy(), w1) <= 1e-12
+        assert rel(h[k, :q].cpu().numpy(), h1) <= 1e-12
+        assert abs(float(nrm2[k]) - np.vdot(w1, w1).real) <= 1e-12 * np.vdot(w1, w1).real
+    assert (h[:, q:] == 7.0).all()
+    with pytest.raises(PeaError):
+        _lib.call("pea_gs_orthogonalize", S, n, 33, _p(V), n, rows * n, _p(wd), n, nb, _p(P),
+                  _p(hw), _p(flags), _p(h), rows + 2, _p(nrm2), K._stream())
+
+
 @pytest.mark.parametrize("precond,shifts", [
     ("line", [complex(0.5, 1.0), complex(0.0, 3.0), complex(2.0, 0.0)]),
     # block-Jacobi only copes when the shift dominates the radial coupling of thin cells
