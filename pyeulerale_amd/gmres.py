@@ -134,8 +134,7 @@ class BatchedGMRES:
         against v_0..v_j, v_{j+1}, column j of H, and its Givens update (``pea_gmres_rotate``)."""
         S, n, R = self.S, st["n"], self.restart
         V, w, z = st["V"], st["w"], st["z"]
-        w.copy_(V[:, j].reshape(w.shape))
-        self.psolve(w, z)
+        self.psolve(V[j].view(w.shape), z)      # basis vector j of every shift: one block
         self.apply(z, w)
         q = j + 1
         H = st["H"]
@@ -144,7 +143,7 @@ class BatchedGMRES:
             # two passes over the basis rows, a third only for shifts whose first projection
             # removed more than half of |w|^2
             h = st["h1"]
-            _lib.call("pea_gs_orthogonalize", S, n, q, _p(V), n, (R + 1) * n, _p(w), n,
+            _lib.call("pea_gs_orthogonalize", S, n, q, _p(V), S * n, n, _p(w), n,
                       st["nb"], _p(st["P"]), _p(st["hw"]), _p(st["flags"]), _p(h), h.shape[1],
                       _p(st["nrm2"]), K._stream())
             hn = st["nrm2"].sqrt()
@@ -154,28 +153,28 @@ class BatchedGMRES:
             # chunks, which a library GEMM of this shape does not do); w -= V h1 together with
             # h2 = V^H w; w -= V h2 together with |w|^2 (pea_gs_update)
             h1, h2 = st["h1"], st["h2"]
-            self._dots(st, V, R + 1, q, w, h1)
-            self._gs_update(st, V, R + 1, q, h1, w, 1, h2)
-            self._gs_update(st, V, R + 1, q, h2, w, 2, st["nrm"])
+            self._dots(st, V, q, w, h1)
+            self._gs_update(st, V, q, h1, w, 1, h2)
+            self._gs_update(st, V, q, h2, w, 2, st["nrm"])
             hn = st["nrm"][:, 0].real.sqrt()
             torch.add(h1[:, :q], h2[:, :q], out=H[:, :q, j])
-        torch.div(w.reshape(S, n), hn.clamp_min(1e-300)[:, None], out=V[:, j + 1])
+        torch.div(w.reshape(S, n), hn.clamp_min(1e-300)[:, None], out=V[j + 1])
         H[:, j + 1, j] = hn.to(C128)
         _lib.call("pea_gmres_rotate", S, R, j, _p(H), _p(st["cs"]), _p(st["sn"]), _p(st["g"]),
                   _p(st["est"]), K._stream())
         st["hist"][j].copy_(st["est"])
 
-    def _dots(self, st, V, rows, q, w, out):
-        """out[s, i] = sum_k conj(V[s, i, k]) w[s, k] for i < q (V with ``rows`` rows per shift)."""
+    def _dots(self, st, V, q, w, out):
+        """out[s, i] = sum_k conj(V[i, s, k]) w[s, k] for i < q (basis V of shape (rows, S, n))."""
         n = st["n"]
-        _lib.call("pea_multidot", self.S, n, q, _p(V), n, rows * n, _p(w), n, st["nb"],
+        _lib.call("pea_multidot", self.S, n, q, _p(V), self.S * n, n, _p(w), n, st["nb"],
                   _p(st["P"]), _p(out), out.shape[1], K._stream())
 
-    def _gs_update(self, st, V, rows, q, h, w, mode, out):
-        """w[s] -= sum_{i<q} V[s, i] h[s, i]; mode 1 also gives out[s, i] = V[s, i]^H w_new,
-        mode 2 out[s, 0] = |w_new[s]|^2 (V with ``rows`` rows per shift)."""
+    def _gs_update(self, st, V, q, h, w, mode, out):
+        """w[s] -= sum_{i<q} V[i, s] h[s, i]; mode 1 also gives out[s, i] = V[i, s]^H w_new,
+        mode 2 out[s, 0] = |w_new[s]|^2 (basis V of shape (rows, S, n))."""
         n = st["n"]
-        _lib.call("pea_gs_update", self.S, n, q, _p(V), n, rows * n, _p(h), h.shape[1], _p(w), n,
+        _lib.call("pea_gs_update", self.S, n, q, _p(V), self.S * n, n, _p(h), h.shape[1], _p(w), n,
                   mode, st["nb"], _p(st["P"]), _p(out), 0 if out is None else out.shape[1],
                   K._stream())
 
@@ -202,7 +201,8 @@ class BatchedGMRES:
         b = b.to(C128).reshape(shape).contiguous()
         n = b[0].numel()
         x = torch.zeros_like(b) if x0 is None else x0.to(C128).reshape(shape).clone()
-        st = dict(n=n, V=torch.empty((S, R + 1, n), dtype=C128, device=dev),
+        # basis (rows, shifts, n): vector j of every shift is one contiguous block
+        st = dict(n=n, V=torch.empty((R + 1, S, n), dtype=C128, device=dev),
                   w=torch.empty(shape, dtype=C128, device=dev),
                   z=torch.empty(shape, dtype=C128, device=dev),
                   H=torch.zeros((S, R + 1, R), dtype=C128, device=dev),
@@ -236,7 +236,7 @@ class BatchedGMRES:
             converged = resid <= self.rtol
             if converged.all() or total >= self.maxiter:
                 break
-            st["V"][:, 0] = r.reshape(S, n) / beta.clamp_min(1e-300)[:, None]
+            st["V"][0] = r.reshape(S, n) / beta.clamp_min(1e-300)[:, None]
             for key in ("H", "cs", "sn", "g"):
                 st[key].zero_()     # fill kernels: these buffers may also be graph inputs
             st["g"][:, 0] = beta.to(C128)
@@ -269,7 +269,7 @@ class BatchedGMRES:
             # V y as 0 - V (-y): one pass over the basis rows
             upd = torch.zeros(shape, dtype=C128, device=dev)
             negY = (-Y[:, :, 0]).contiguous()
-            self._gs_update(st, st["V"], R + 1, j_done, negY, upd, 0, None)
+            self._gs_update(st, st["V"], j_done, negY, upd, 0, None)
             self.psolve(upd, st["z"])
             x += st["z"]
             restarts += 1
