@@ -489,7 +489,8 @@ def gmres_steps(torch, K, S: int = 32, steps: int = 30):
     """Batched GMRES throughput at 1025x513 (where the exact ring LU does not fit one GPU):
     30 Arnoldi steps (graph-replayed after the first cycle, radial-line preconditioner, block
     operator) for 32 shifts s = i*logspace(1, 2, 32) at once, against the same steps for one
-    shift at a time.  Time per shift-step; convergence is not the point here."""
+    shift at a time; and the batched steps with the matrix-free operator (all 32 tangents in
+    one launch).  Time per shift-step; convergence is not the point here."""
     from pyeulerale_amd.gmres import BatchedGMRES
     X, V, U, M, N = make_case(1025, 513, 0.04, 0.5, 0.0, 0.0, 0.0, 30.0, 1)
     g, p = K.Grid(M, N), K.Params(aoa_deg=0.0)
@@ -500,8 +501,9 @@ def gmres_steps(torch, K, S: int = 32, steps: int = 30):
     gen = torch.Generator(device="cuda").manual_seed(0)
     b = torch.randn((S, *g.cell_shape), dtype=torch.complex128, device="cuda", generator=gen)
 
-    def run(sh, rhs):
-        gm = BatchedGMRES(g, JU, sh, restart=steps // 2, maxiter=steps, rtol=0.0)
+    def run(sh, rhs, operator="blocks"):
+        gm = BatchedGMRES(g, JU, sh, restart=steps // 2, maxiter=steps, rtol=0.0,
+                          operator=operator, point=(p, 0.5, g, Xd, Vd, Ud))
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         gm.solve(rhs)
@@ -510,10 +512,13 @@ def gmres_steps(torch, K, S: int = 32, steps: int = 30):
     run(shifts[:1], b[:1])                         # library warm-up
     t_b = run(shifts, b)
     t_1 = run(shifts[:1], b[:1])
+    run(shifts[:1], b[:1], "jvp")
+    t_mf = run(shifts, b, "jvp")
     return {"grid": "1025x513", "shifts": S, "arnoldi_steps": steps,
             "ms_per_shift_step_batched": t_b / (S * steps) * 1e3,
             "ms_per_shift_step_single": t_1 / steps * 1e3,
-            "speedup_batched_vs_one_shift_at_a_time": (t_1 * S) / t_b}
+            "speedup_batched_vs_one_shift_at_a_time": (t_1 * S) / t_b,
+            "ms_per_shift_step_batched_matrix_free": t_mf / (S * steps) * 1e3}
 
 
 def cpu_baseline(seconds, X, V, U, mach, aoa):

@@ -118,7 +118,9 @@ int pea_jvp_odes(const pea_params *prm, double mach, int M, int N, const double 
 
 /* k directions at once: dU, dX, dV are k consecutive complex arrays (stride one full array),
  * dmach k complex values; dR receives k complex arrays.  Any direction pointer may be NULL.
- * k in {1, 2, 4, 8}. */
+ * With dU only: one launch of the face kernel, one direction per grid z, any k in 1..65535,
+ * each output bitwise equal to pea_jvp_odes of that direction.  Otherwise k in {1, 2, 4, 8}.
+ * The reference applies one direction per call (apply_odes_wrt_states_fwd, f90:825-865). */
 int pea_jvp_odes_multi(const pea_params *prm, double mach, int M, int N, int k,
                        const double *X, const double *V, const double *U, const double *dmach,
                        const double *dX, const double *dV, const double *dU, double *dR,
@@ -140,6 +142,14 @@ int pea_jvp_forces(int M, int N, const double *X, const double *U, const double 
 int pea_vjp_odes(const pea_params *prm, double mach, int M, int N, const double *X,
                  const double *V, const double *U, const double *w, double *gU, double *gX,
                  double *gV, double *gmach, void *workspace, void *stream);
+
+/* k seeds at once, states output only: w and gU are k consecutive complex arrays.  One launch of
+ * the face adjoint (one seed per grid z) where N >= 18, else one adjoint per seed; each gU
+ * bitwise equal to pea_vjp_odes of that seed.  The reference transposes one product per call
+ * (apply_odes_wrt_states_rev, f90:875-918). */
+int pea_vjp_odes_states_multi(const pea_params *prm, double mach, int M, int N, int k,
+                              const double *X, const double *V, const double *U, const double *w,
+                              double *gU, void *stream);
 long long pea_vjp_workspace_bytes(int M, int N);
 
 /* gU = (dF/dU)^T wF, gX = (dF/dX)^T wF (complex; either may be NULL).  Replaces

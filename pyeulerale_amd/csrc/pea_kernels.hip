@@ -142,6 +142,12 @@ struct LdReal {
         w[1] = buf_load(r, p, g.P);
     }
     PEA_HD double machv() const { return mach; }
+    // item b of a batch launch (grid z): the b-th of consecutive state fields
+    PEA_HD LdReal at(int b) const {
+        LdReal r = *this;
+        r.U += (long)b * 4 * g.C;
+        return r;
+    }
 };
 
 // K tangent lanes: lane j is the real (j even) or imaginary (j odd) part of complex direction
@@ -174,6 +180,7 @@ struct LdDual {
     }
     PEA_HD void vertex(int p, Dual<K>* x) const { vec2(X, dX, p, x); }
     PEA_HD void velocity(int p, Dual<K>* w) const { vec2(V, dV, p, w); }
+    PEA_HD LdDual at(int) const { return *this; }   // not launched as a batch
     PEA_HD Dual<K> machv() const {
         Dual<K> r;
         r.v = mach;
@@ -216,6 +223,12 @@ struct LdDualU {
         w[1] = buf_load(r, p, g.P);
     }
     PEA_HD double machv() const { return mach; }
+    // item b of a batch launch (grid z): the b-th group of K/2 consecutive complex directions
+    PEA_HD LdDualU at(int b) const {
+        LdDualU r = *this;
+        r.dU += (long)b * (K / 2) * 8 * g.C;
+        return r;
+    }
 };
 
 // Complex step on the states only (geometry and Mach real).
@@ -429,8 +442,7 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const FsDir& 
     __shared__ double sU[4 * W * BOX];
     __shared__ double sFb[4 * W * SB], sFi[4 * W * TN];
     const Grid g = ld.g;
-    Ld l = ld;
-    l.U = ld.U + (long)bz * 4 * g.C;
+    const Ld l = ld.at(bz);
     const int m0 = bx * TM, n0 = by * TN;
     const int tid = threadIdx.x;
     const int tm = tid % TM, tn = tid / TM;
@@ -654,13 +666,16 @@ template <int K>
 struct StoreDualB {
     double *dR, *R;
     int C;
-    PEA_HD void put(int, int c, const Dual<K>* r) const {
+    // item b of a batch writes the b-th group of K/2 complex outputs; R (the primal, the same
+    // for every item) only from item 0
+    PEA_HD void put(int b, int c, const Dual<K>* r) const {
+        double* out = dR + (long)b * (K / 2) * 8 * C;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const long e = (long)v * C + c;
-            if (R) R[e] = r[v].v;
+            if (R && b == 0) R[e] = r[v].v;
 #pragma unroll
-            for (int j = 0; j < K; ++j) dR[2 * ((long)(j >> 1) * 4 * C + e) + (j & 1)] = r[v].d[j];
+            for (int j = 0; j < K; ++j) out[2 * ((long)(j >> 1) * 4 * C + e) + (j & 1)] = r[v].d[j];
         }
     }
 };
@@ -1616,7 +1631,8 @@ k_face_vjp(pea_params P, FsDir fs, LdReal ld, const double* __restrict__ wseed,
            double* __restrict__ gU) {
     const unsigned tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y,
                                    gridDim.x * gridDim.y);   // XCD-contiguous tiles
-    face_vjp_tile<TM, TN, NT>(P, fs, ld, wseed, gU, (int)(tile % gridDim.x) * TM,
+    const long z = (long)blockIdx.z * 8 * ld.g.C;            // seed z of a batch
+    face_vjp_tile<TM, TN, NT>(P, fs, ld, wseed + z, gU + z, (int)(tile % gridDim.x) * TM,
                               (int)(tile / gridDim.x) * TN);
 }
 
@@ -2045,9 +2061,17 @@ int pea_jvp_odes_multi(const pea_params* prm, double mach, int M, int N, int k,
     const int bs = cell_block(g.C);
     const unsigned nb = blocks_for(g.C, bs);
     hipStream_t st = as_stream(stream);
+    const bool u_only = dU && !dX && !dV && !dmach;
+    if (u_only) {   // one face-kernel launch, one complex direction per grid z
+        if (k < 1 || k > 65535) return fail(PEA_ERR_ARG, "k must be 1..65535 (got %d)", k);
+        k_face_fwd<Dual<2>, double, kJvpTM, kJvpTN><<<fwd_grid(g, k, kJvpTM, kJvpTN),
+                                                      kJvpTM * kJvpTN, 0, st>>>(
+            *prm, fs_dir(*prm), LdDualU<2>{X, V, U, dU, mach, g}, StoreDualB<2>{dR, nullptr, g.C});
+        PEA_LAUNCH_CHECK("k_face_fwd");
+        return PEA_OK;
+    }
     if (k != 1 && k != 2 && k != 4 && k != 8)
         return fail(PEA_ERR_ARG, "k must be 1, 2, 4 or 8 (got %d)", k);
-    const bool u_only = dU && !dX && !dV && !dmach;
     // directions are processed two at a time (four tangent lanes per pass)
     const long su = 2L * 4 * 2 * g.C, sx = 2L * 2 * 2 * g.P;
     for (int h = 0; h < k; h += 2) {
@@ -2100,6 +2124,30 @@ int pea_vjp_odes(const pea_params* prm, double mach, int M, int N, const double*
     if (!prm || !X || !V || !U || !w) return fail(PEA_ERR_NULL, "null argument");
     return launch_vjp<double>(*prm, LdReal{X, V, U, mach, make_grid(M, N)}, w, gU, gX, gV,
                               gmach, workspace, as_stream(stream));
+}
+
+int pea_vjp_odes_states_multi(const pea_params* prm, double mach, int M, int N, int k,
+                              const double* X, const double* V, const double* U, const double* w,
+                              double* gU, void* stream) {
+    if (int rc = check_dims(M, N)) return rc;
+    if (!prm || !X || !V || !U || !w || !gU) return fail(PEA_ERR_NULL, "null argument");
+    if (k < 1 || k > 65535) return fail(PEA_ERR_ARG, "k must be 1..65535 (got %d)", k);
+    const Grid g = make_grid(M, N);
+    const LdReal ld{X, V, U, mach, g};
+    hipStream_t st = as_stream(stream);
+    if (N >= kVjpFaceTN + 3 && g_face_vjp) {   // one launch, one seed per grid z
+        const dim3 grid((M + kVjpFaceTM - 1) / kVjpFaceTM, (N + kVjpFaceTN - 1) / kVjpFaceTN, k);
+        k_face_vjp<kVjpFaceTM, kVjpFaceTN, 256><<<grid, 256, 0, st>>>(*prm, fs_dir(*prm), ld, w, gU);
+        PEA_LAUNCH_CHECK("k_face_vjp");
+        return PEA_OK;
+    }
+    for (int i = 0; i < k; ++i) {   // grids too narrow for the face adjoint's tile
+        const long o = (long)i * 8 * g.C;
+        if (int rc = launch_vjp<double>(*prm, ld, w + o, gU + o, nullptr, nullptr, nullptr,
+                                        nullptr, st))
+            return rc;
+    }
+    return PEA_OK;
 }
 
 int pea_vjp_forces(int M, int N, const double* X, const double* U, const double* wF,

@@ -4,8 +4,9 @@ The reference solves one shift at a time with SuperLU on a freshly assembled CSR
 (core.py:512-599).  Here all S shifts share one linearization point and advance together:
 
 * operator: ``"blocks"`` -- the assembled complex-step blocks applied to all S vectors in one
-  launch (``pea_block_multi``, blocks read once per batch); or ``"jvp"`` -- the hand-written
-  tangent (adjoint for the transposed system) per vector, no assembled matrix;
+  launch (``pea_block_multi``, blocks read once per batch); or ``"jvp"`` -- matrix-free, the
+  hand-written tangent (adjoint for the transposed system) of all S vectors in one batched
+  launch (``pea_jvp_odes_multi`` / ``pea_vjp_odes_states_multi``), no assembled matrix;
 * right preconditioning per shift: ``"line"`` (block LU of every radial line,
   ``pea_line_factor``/``pea_line_solve``), ``"jacobi"`` (4x4 diagonal blocks) or ``None``;
 * orthogonalisation: classical Gram-Schmidt with selective reorthogonalisation (the projection is
@@ -71,6 +72,8 @@ class BatchedGMRES:
         dev = JU.device if JU is not None else point[5].device
         self.device = dev
         self.sh = torch.as_tensor(self.shifts, dtype=C128, device=dev).contiguous()
+        # -s (adjoint: -conj(s)) per vector, broadcast over a cell field
+        self._neg_shift = -(self.sh.conj() if self.adjoint else self.sh).view(-1, 1, 1, 1)
         if operator == "blocks" and JU is None:
             raise ValueError("operator 'blocks' needs the assembled blocks")
         if operator == "jvp" and point is None:
@@ -107,14 +110,14 @@ class BatchedGMRES:
             _lib.call("pea_block_multi", g.M, g.N, self.S, _p(self.JU), _p(self.sh),
                       int(self.adjoint), _p(x), _p(y), K._stream())
             return y
+        # matrix-free: all S products in one batched launch of the face tangent (adjoint), then
+        # y -= s x with s per vector
         p, mach, gg, X, V, U = self.point
-        for k in range(self.S):
-            if self.adjoint:
-                K.vjp_odes(p, mach, g, X, V, U, x[k], out={"states": y[k]})
-                y[k].sub_(x[k] * complex(np.conj(self.shifts[k])))
-            else:
-                K.jvp_odes(p, mach, g, X, V, U, dU=x[k], dR=y[k])
-                y[k].sub_(x[k] * complex(self.shifts[k]))
+        if self.adjoint:
+            K.vjp_odes_states_multi(p, mach, g, X, V, U, x, gU=y)
+        else:
+            K.jvp_odes_multi(p, mach, g, X, V, U, dU=x, dR=y)
+        y.addcmul_(x, self._neg_shift)
         return y
 
     def psolve(self, r: torch.Tensor, z: torch.Tensor) -> torch.Tensor:

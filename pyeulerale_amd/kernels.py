@@ -241,7 +241,8 @@ def jvp_odes(p: Params, mach: float, g: Grid, X, V, U, dmach=None, dX=None, dV=N
 
 def jvp_odes_multi(p: Params, mach: float, g: Grid, X, V, U, dU=None, dX=None, dV=None,
                    dmach=None, dR=None) -> torch.Tensor:
-    """k directions at once; direction tensors have a leading axis k (k in 1, 2, 4, 8)."""
+    """k directions at once; direction tensors have a leading axis k.  States-only directions
+    (dU alone) take one batched launch of the face kernel and any k; otherwise k in 1, 2, 4, 8."""
     g.check_primal(X, V, U)
     k = next(t.shape[0] for t in (dU, dX, dV) if t is not None)
     _opt(dU, (k, *g.cell_shape), C128, "d_states")
@@ -255,6 +256,18 @@ def jvp_odes_multi(p: Params, mach: float, g: Grid, X, V, U, dU=None, dX=None, d
     _lib.call("pea_jvp_odes_multi", ctypes.byref(p.c()), float(mach), g.M, g.N, k, _ptr(X),
               _ptr(V), _ptr(U), _ptr(dm), _ptr(dX), _ptr(dV), _ptr(dU), _ptr(dR), _stream())
     return dR
+
+
+def vjp_odes_states_multi(p: Params, mach: float, g: Grid, X, V, U, w, gU=None) -> torch.Tensor:
+    """(dR/dU)^T w for k seeds at once (w of shape (k, 4, N, M), complex): one batched launch of
+    the face adjoint."""
+    g.check_primal(X, V, U)
+    k = w.shape[0]
+    _need(w, (k, *g.cell_shape), C128, "seeds")
+    gU = torch.empty_like(w) if gU is None else _need(gU, (k, *g.cell_shape), C128, "d_states")
+    _lib.call("pea_vjp_odes_states_multi", ctypes.byref(p.c()), float(mach), g.M, g.N, k,
+              _ptr(X), _ptr(V), _ptr(U), _ptr(w), _ptr(gU), _stream())
+    return gU
 
 
 def jvp_forces(g: Grid, X, U, dX=None, dU=None, dF=None) -> torch.Tensor:

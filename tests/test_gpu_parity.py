@@ -143,6 +143,40 @@ def test_jvp_vjp_against_complex_step_blocks(name, orc, K):
     assert torch.equal(only["states"], again["states"])
 
 
+@pytest.mark.parametrize("name", ["tiny", "small_pitch", "narrow"])
+def test_jvp_multi_directions(name, orc, K):
+    """k directions per call.  States-only directions (one batched launch of the face kernel,
+    k = 5) are bitwise equal to one pea_jvp_odes per direction and match the oracle's
+    complex-step blocks; mixed-input directions (k = 2, the cell-based path) match single
+    products to round-off; k adjoint seeds in one launch equal one pea_vjp_odes per seed."""
+    X, V, U, mach, aoa = setup_case(name)
+    g, p, Xd, Vd, Ud = _dev(K, X, V, U, aoa)
+    J = _linearization(orc, X, V, U, mach, aoa)
+    rng = np.random.default_rng(11)
+    k = 5
+    dUs = [crandn(rng, U.shape) for _ in range(k)]
+    dUd = torch.stack([dev_cells(d) for d in dUs])
+    out = K.jvp_odes_multi(p, mach, g, Xd, Vd, Ud, dU=dUd)
+    for i in range(k):
+        one = K.jvp_odes(p, mach, g, Xd, Vd, Ud, dU=dUd[i].contiguous())
+        assert torch.equal(out[i], one), i
+        ref = orc.apply_odes_wrt_states_fwd(J["U"], dUs[i])
+        assert rel(host(out[i]), ref) <= TOL_JAC
+    dXs = torch.stack([dev_verts(crandn(rng, X.shape)) for _ in range(2)])
+    out2 = K.jvp_odes_multi(p, mach, g, Xd, Vd, Ud, dU=dUd[:2].contiguous(), dX=dXs)
+    for i in range(2):
+        one = K.jvp_odes(p, mach, g, Xd, Vd, Ud, dU=dUd[i].contiguous(), dX=dXs[i].contiguous())
+        assert rel(host(out2[i]), host(one)) <= 1e-13
+    # k seeds of the states adjoint: one launch of the face adjoint for N >= 18, one adjoint
+    # per seed below (tiny: N = 8)
+    gs = K.vjp_odes_states_multi(p, mach, g, Xd, Vd, Ud, dUd)
+    for i in range(k):
+        one = K.vjp_odes(p, mach, g, Xd, Vd, Ud, dUd[i].contiguous(), want=("states",))["states"]
+        assert torch.equal(gs[i], one), i
+        ref = orc.apply_odes_wrt_states_rev(J["U"], dUs[i])
+        assert rel(host(gs[i]), ref) <= TOL_JAC
+
+
 @pytest.mark.parametrize("name", ["tiny", "small_pitch"])
 def test_complex_step_path(name, orc, K):
     X, V, U, mach, aoa = setup_case(name)
