@@ -236,6 +236,10 @@ int pea_line_factor(int M, int N, int S, const double *JU, const double *shifts,
                     int transposed, double *F, void *stream);
 int pea_line_solve(int M, int N, int S, const double *JU, const double *F, int transposed,
                    const double *r, double *x, void *stream);
+/* Lanes per radial line in pea_line_solve: 4 (one per block row) or 16 (one per block element);
+ * 0 (the default) picks 4 when S*N >= 32768, else 16.  Process-wide; both give the same
+ * solution to round-off.  No counterpart in the reference (it factorises with SuperLU). */
+int pea_line_solve_lanes(int lanes);
 
 /* Exact direct resolvent by block LU over radial rings (replaces the SuperLU factorisation of
  * core.py:562-564 and 596-598).  Ring k holds every cell at radial index k; its 4N unknowns

@@ -67,6 +67,7 @@ _SIGS = {
     "pea_line_factor_bytes": [_I, _I, _I],
     "pea_line_factor": [_I, _I, _I, _P, _P, _I, _P, _P],
     "pea_line_solve": [_I, _I, _I, _P, _P, _I, _P, _P, _P],
+    "pea_line_solve_lanes": [_I],
     "pea_jacobi_factor": [_I, _I, _I, _P, _P, _I, _P, _P],
     "pea_jacobi_solve": [_I, _I, _I, _P, _P, _P, _P],
     "pea_ring_band": [_I, _I, _I, _P, _P, _I, _I, _I, _P, _P],
@@ -85,7 +86,7 @@ _RESTYPES = {"pea_last_error": _c.c_char_p, "pea_vjp_workspace_bytes": _c.c_long
              "pea_multi_workspace_bytes": _c.c_longlong,
              "pea_resolvent_last_error": _c.c_char_p, "pea_line_factor_bytes": _c.c_longlong}
 
-_RESOLVENT = {"pea_block_multi", "pea_line_factor", "pea_line_solve", "pea_jacobi_factor",
+_RESOLVENT = {"pea_block_multi", "pea_line_factor", "pea_line_solve", "pea_line_solve_lanes", "pea_jacobi_factor",
               "pea_jacobi_solve", "pea_ring_band", "pea_ring_couple", "pea_dense_lu_factor",
               "pea_dense_lu_solve", "pea_gmres_rotate",
               "pea_multidot", "pea_gs_update", "pea_gs_orthogonalize"}

@@ -7,8 +7,8 @@
 // * k_line_factor / k_line_solve: radial-line preconditioner.  For every angular index n and
 //   shift s the block-pentadiagonal matrix of the radial line (blocks ii, i, c - sI, o, oo) is
 //   factorised by block LU (partial pivoting inside each 4x4 diagonal block) and solved.
-//   Factorisation: one thread per (line, shift), m sequential; solve: four lanes per (line,
-//   shift), one per row of the 4x4 blocks.  The transposed variant factorises the line of
+//   Factorisation: one thread per (line, shift), m sequential; solve: sixteen lanes per (line,
+//   shift), one per element of the 4x4 blocks.  The transposed variant factorises the line of
 //   (dR/dU)^T - conj(s) I for the adjoint solve.
 // * k_jacobi_factor / k_jacobi_solve: block-Jacobi preconditioner (4x4 diagonal blocks).
 // * k_ring_band / k_ring_couple: pieces of the exact block-banded LU over radial rings.  A ring
@@ -142,27 +142,27 @@ __device__ __forceinline__ B4 load_jblock(const double* JU, int C, int cell, int
     return r;
 }
 
-// factor storage: lines in groups of kLineGroup (the lines one wave of the solve holds); per
-// (shift, group) the steps m one after another, each with its 4 slots x 16 elements x
-// kLineGroup lines:  F[((((sh*NG + g)*M + m)*4 + slot)*16 + 4i + j)*kLineGroup + n % kLineGroup]
-// with g = n / kLineGroup, slots 0 = L2, 1 = L1, 2 = Dinv, 3 = U1.  A wave thus streams through
-// one contiguous region as it walks the line, instead of touching 64 planes spread over the
-// whole array at every step.
+// factor storage: lines in groups of kLineGroup (the lines one workgroup of the solve holds);
+// per (shift, group) the steps m one after another, each with its 4 slots x kLineGroup lines x
+// 16 elements:  F[((((sh*NG + g)*M + m)*4 + slot)*kLineGroup + n % kLineGroup)*16 + 4i + j] with
+// g = n / kLineGroup, slots 0 = L2, 1 = L1, 2 = Dinv, 3 = U1.  One block of a workgroup's lines
+// is a contiguous 4 KB (1 KB per wave), and the workgroup streams through one contiguous region
+// as it walks its lines.
 constexpr int kLineGroup = 16;
 __host__ __device__ __forceinline__ long line_groups(int N) {
     return (N + kLineGroup - 1) / kLineGroup;
 }
 __device__ __forceinline__ long fidx(int M, int N, int sh, int m, int slot, int ij, int n) {
     const long g = n / kLineGroup, nl = n % kLineGroup;
-    return (((((long)sh * line_groups(N) + g) * M + m) * 4 + slot) * 16 + ij) * kLineGroup + nl;
+    return (((((long)sh * line_groups(N) + g) * M + m) * 4 + slot) * kLineGroup + nl) * 16 + ij;
 }
 // after the S shifts' complex factors (in doubles): the real block U2 = (m, m+2) of the system,
-// the same for every shift, element (i, j) of line n at U2[((g*M + m)*16 + 4i + j)*kLineGroup
-// + n % kLineGroup]
+// the same for every shift, element (i, j) of line n at U2[((g*M + m)*kLineGroup + n %
+// kLineGroup)*16 + 4i + j]
 __device__ __forceinline__ long u2idx(int M, int N, int S, int m, int ij, int n) {
     const long base = 2L * S * line_groups(N) * M * 4 * 16 * kLineGroup;
     const long g = n / kLineGroup, nl = n % kLineGroup;
-    return base + (((long)g * M + m) * 16 + ij) * kLineGroup + nl;
+    return base + (((long)g * M + m) * kLineGroup + nl) * 16 + ij;
 }
 // after U2 (in doubles): the system's line blocks (m, m + off), off = -2..2, without the shift,
 // lines fastest: SB[((off + 2)*16 + 4i + j)*M*N + m*N + n], written by k_line_blocks for the
@@ -282,6 +282,177 @@ k_line_factor(int M, int N, int S, const double* __restrict__ sh, int tr, double
     }
 }
 
+// Orders the LDS accesses of a wave's lanes: what one lane wrote, another lane reads after this.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// A double moved between lanes of a quad by DPP (quad_perm control CTRL).
+template <int CTRL>
+__device__ __forceinline__ double quad_dpp(double x) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(
+        (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo));
+}
+// Sum over the quad, the same bits in all four lanes: (x0 + x1) + (x2 + x3) in every lane,
+// addition being commutative.  quad_perm [1,0,3,2] = 0xB1, [2,3,0,1] = 0x4E.
+__device__ __forceinline__ double quad_sum(double x) {
+    x += quad_dpp<0xB1>(x);
+    return x + quad_dpp<0x4E>(x);
+}
+__device__ __forceinline__ Z quad_sum(Z x) { return Z{quad_sum(x.r), quad_sum(x.i)}; }
+
+// x = (line LU)^-1 r for every line and shift.  r and x: S consecutive complex cell fields
+// (4 planes of M x N each); x may alias r.  Sixteen lanes per (line, shift), lane 4i + j holding
+// element (i, j) of the factor blocks and component j of the previous steps' vectors; a wave
+// holds 4 lines and a workgroup the kLineGroup = 16 lines of one factor group.  A row's sum over
+// j is a quad butterfly (DPP), which leaves it in the four lanes of row i; component j then
+// moves to the lanes of column j by one transpose within the line's 16 lanes (ds_bpermute).
+// The workgroup's waves stay in step (a barrier per block of steps), so together they read each
+// step's factors as one contiguous 4 KB per block.
+// * The vectors run along m, the fastest index of the cell fields: each wave moves them through
+//   LDS in blocks of kLineBlock steps, one whole 128 B line per (line, row).
+// * The factor loads of a step do not depend on the recurrence: they are issued kLineAhead
+//   steps early into a ring of register sets (the block loop is unrolled, so every set is
+//   addressed at compile time).
+constexpr int kLineThreads = 256;
+constexpr int kLineAhead = 4;
+int g_line_lanes = 0;   // lanes per line of the solve: 0 automatic, 4 or 16 (pea_line_solve_lanes)
+constexpr int kLineBlock = 8;        // 8 steps x 16 B: one 128 B line per (line, row)
+constexpr int kLinePad = kLineBlock + 1;
+
+__global__ void __launch_bounds__(kLineThreads)
+k_line_solve_elems(int M, int N, int S, const double* __restrict__ F, const double* r, double* x) {
+    static_assert(kLineThreads / 16 == kLineGroup, "a workgroup holds one line group");
+    static_assert(kLineBlock % kLineAhead == 0, "ring slots repeat within a block");
+    __shared__ double2 sv[kLineThreads / 64][16][kLinePad];   // per wave: 4 lines x 4 rows
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int n0 = blockIdx.x * kLineGroup + wv * 4;   // first line of the wave
+    const int nl = lane >> 4, ij = lane & 15, i = ij >> 2, j = ij & 3;
+    const int n = n0 + nl, s = blockIdx.y;
+    // every wave runs to the end (barriers): lines past N, inside the last, padded group,
+    // compute values that are never stored
+    if (s >= S) return;
+    const int tsrc = (lane & ~15) | (j << 2) | i;   // lane of row j, column i: the transpose
+    const long C = (long)M * N;
+    const double2* F2 = reinterpret_cast<const double2*>(F);
+    const double2* r2 = reinterpret_cast<const double2*>(r);
+    double2* x2 = reinterpret_cast<double2*>(x);
+    double2(*buf)[kLinePad] = sv[wv];
+    const Z zero = {0.0, 0.0};
+    const int prow = nl * 4 + i;   // (line, row) pair of this lane in the staging block
+    // block transfer: element e = lane + 64k of the wave's 16 x kLineBlock block is step e % 8
+    // of pair e / 8, the pair being (line, row) = (e / 32, e / 8 % 4)
+    auto vload = [&](const double2* a, int m0) {
+#pragma unroll
+        for (int k = 0; k < 16 * kLineBlock / 64; ++k) {
+            const int e = lane + 64 * k, p = e / kLineBlock, q = e % kLineBlock;
+            const int pn = n0 + p / 4, m = m0 + q;
+            if (pn < N && m < M) buf[p][q] = a[(long)s * 4 * C + (long)(p & 3) * C + (long)pn * M + m];
+        }
+    };
+    auto vstore = [&](double2* a, int m0) {
+#pragma unroll
+        for (int k = 0; k < 16 * kLineBlock / 64; ++k) {
+            const int e = lane + 64 * k, p = e / kLineBlock, q = e % kLineBlock;
+            const int pn = n0 + p / 4, m = m0 + q;
+            if (pn < N && m < M) a[(long)s * 4 * C + (long)(p & 3) * C + (long)pn * M + m] = buf[p][q];
+        }
+    };
+    auto fel = [&](int m, int slot) {   // element (i, j) of factor block (m, slot)
+        const double2 v = F2[fidx(M, N, s, m, slot, ij, n)];
+        return Z{v.x, v.y};
+    };
+    auto tr = [&](Z a) {   // component j, from the lanes of row j
+        return Z{__shfl(a.r, tsrc), __shfl(a.i, tsrc)};
+    };
+    // forward: y_m = r_m - L2 y_{m-2} - L1 y_{m-1}  (y stored in x)
+    {
+        Z y1 = zero, y2 = zero;   // component j of y_{m-1}, y_{m-2}
+        Z l2[kLineAhead], l1[kLineAhead];
+        auto load = [&](int m, int k) {
+            if (m < M) {
+                if (m >= 2) l2[k] = fel(m, 0);
+                if (m >= 1) l1[k] = fel(m, 1);
+            }
+        };
+#pragma unroll
+        for (int k = 0; k < kLineAhead; ++k) load(k, k);
+        for (int m0 = 0; m0 < M; m0 += kLineBlock) {
+            vload(r2, m0);
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < kLineBlock; ++k) {
+                const int m = m0 + k, slot = k % kLineAhead;
+                if (m < M) {
+                    Z acc = zero;
+                    if (j == 0) {
+                        const double2 rv = buf[prow][k];
+                        acc = Z{rv.x, rv.y};
+                    }
+                    if (m >= 2) acc = zsub(acc, zmul(l2[slot], y2));
+                    if (m >= 1) acc = zsub(acc, zmul(l1[slot], y1));
+                    const Z y = quad_sum(acc);            // row i of y_m, in all four lanes
+                    if (j == 0) buf[prow][k] = make_double2(y.r, y.i);
+                    y2 = y1;
+                    y1 = tr(y);
+                }
+                load(m + kLineAhead, slot);
+            }
+            __syncthreads();
+            vstore(x2, m0);
+            __syncthreads();
+        }
+    }
+    // backward: x_m = Dinv (y_m - U1 x_{m+1} - U2 x_{m+2}),  U2 = block (m, m+2) of the system
+    {
+        Z x1 = zero, xx2 = zero;   // component j of x_{m+1}, x_{m+2}
+        Z u1[kLineAhead], di[kLineAhead];
+        double u2[kLineAhead];
+        auto load = [&](int m, int k) {
+            if (m >= 0 && m < M) {
+                if (m + 1 < M) u1[k] = fel(m, 3);
+                if (m + 2 < M) u2[k] = F[u2idx(M, N, S, m, ij, n)];
+                di[k] = fel(m, 2);
+            }
+        };
+        const int mt = (M - 1) / kLineBlock * kLineBlock;   // first step of the top block
+        // step m = m0 + kLineBlock - 1 - k of a block uses ring slot m % kLineAhead
+#pragma unroll
+        for (int k = 0; k < kLineAhead; ++k)
+            load(mt + kLineBlock - 1 - k, (kLineBlock - 1 - k) % kLineAhead);
+        for (int m0 = mt; m0 >= 0; m0 -= kLineBlock) {
+            vload(x2, m0);
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < kLineBlock; ++k) {
+                const int q = kLineBlock - 1 - k, m = m0 + q, slot = q % kLineAhead;
+                if (m < M) {
+                    Z acc = zero;
+                    if (j == 0) {
+                        const double2 yv = buf[prow][q];
+                        acc = Z{yv.x, yv.y};
+                    }
+                    if (m + 1 < M) acc = zsub(acc, zmul(u1[slot], x1));
+                    if (m + 2 < M) acc = zsub(acc, zmul(Z{u2[slot], 0.0}, xx2));
+                    const Z a = tr(quad_sum(acc));          // component j of y_m - U1 x - U2 x
+                    const Z xm = quad_sum(zmul(di[slot], a));   // row i of x_m
+                    if (j == 0) buf[prow][q] = make_double2(xm.r, xm.i);
+                    xx2 = x1;
+                    x1 = tr(xm);
+                }
+                load(m - kLineAhead, slot);
+            }
+            __syncthreads();
+            vstore(x2, m0);
+            __syncthreads();
+        }
+    }
+}
+
 // Lane J of each quad of lanes, broadcast to the quad (DPP quad_perm [J, J, J, J]).
 template <int J>
 __device__ __forceinline__ double quad_bcast(double x) {
@@ -300,26 +471,13 @@ __device__ __forceinline__ void quad_gather(Z mine, Z* y) {
     y[3] = Z{quad_bcast<3>(mine.r), quad_bcast<3>(mine.i)};
 }
 
-// Orders the LDS accesses of a wave's lanes: what one lane wrote, another lane reads after this.
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
-
-// x = (line LU)^-1 r for every line and shift.  r and x: S consecutive complex cell fields
-// (4 planes of M x N each); x may alias r.  Four lanes per (line, shift): lane i holds row i of
-// the 4-vectors and of the factor blocks, and the whole vectors of the two previous steps are
-// shared in the quad by DPP broadcasts.  A wave holds kLineGroup lines.
-// * The vectors run along m, which is the fastest index of the cell fields, so a lane's step
-//   would touch 16 B of its own cache line: the wave moves them through LDS in blocks of
-//   kLineBlock steps instead, whole lines at a time.
-// * The factor loads of a step do not depend on the recurrence: they are issued kLineAhead
-//   steps early into a ring of register sets (the block loop is unrolled, so every set is
-//   addressed at compile time).
-constexpr int kLineThreads = 256;
-constexpr int kLineAhead = 4;
-constexpr int kLineBlock = 8;        // 8 steps x 16 B: one 128 B line per (line, row)
-constexpr int kLinePad = kLineBlock + 1;
+// The same solve with four lanes per (line, shift), for many lines and shifts: lane i holds row
+// i of the 4-vectors and of the factor blocks (elements 4i..4i+3, contiguous), and the whole
+// vectors of the two previous steps are shared in the quad by DPP broadcasts.  A wave holds the
+// kLineGroup lines of one factor group and reads a block of them as one contiguous 4 KB.
+// Vectors move through LDS as in k_line_solve_elems; factor loads run kRowsAhead steps early
+// (two: its row operands need 4x the registers of an element's).
+constexpr int kRowsAhead = 2;
 
 struct LineFwd {   // operands of one forward step
     Z l2[4], l1[4];
@@ -330,9 +488,9 @@ struct LineBwd {   // operands of one backward step
 };
 
 __global__ void __launch_bounds__(kLineThreads)
-k_line_solve(int M, int N, int S, const double* __restrict__ F, const double* r, double* x) {
+k_line_solve_rows(int M, int N, int S, const double* __restrict__ F, const double* r, double* x) {
     static_assert(kLineThreads % 64 == 0 && 64 / 4 == kLineGroup, "a wave holds one line group");
-    static_assert(kLineBlock % kLineAhead == 0, "ring slots repeat within a block");
+    static_assert(kLineBlock % kRowsAhead == 0, "ring slots repeat within a block");
     __shared__ double2 sv[kLineThreads / 64][64][kLinePad];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int n0 = (blockIdx.x * kLineThreads + wv * 64) / 4;   // first line of the wave
@@ -378,16 +536,16 @@ k_line_solve(int M, int N, int S, const double* __restrict__ F, const double* r,
                 if (m >= 1) frow(m, 1, f.l1);
             }
         };
-        LineFwd ring[kLineAhead];
+        LineFwd ring[kRowsAhead];
 #pragma unroll
-        for (int k = 0; k < kLineAhead; ++k) load(k, ring[k]);
+        for (int k = 0; k < kRowsAhead; ++k) load(k, ring[k]);
         for (int m0 = 0; m0 < M; m0 += kLineBlock) {
             vload(r2, m0);
             wave_sync();
 #pragma unroll
             for (int k = 0; k < kLineBlock; ++k) {
                 const int m = m0 + k;
-                LineFwd& f = ring[k % kLineAhead];
+                LineFwd& f = ring[k % kRowsAhead];
                 if (m < M) {
                     const double2 rv = buf[lane][k];
                     Z acc = {rv.x, rv.y};
@@ -406,7 +564,7 @@ k_line_solve(int M, int N, int S, const double* __restrict__ F, const double* r,
                         y1[j] = y[j];
                     }
                 }
-                load(m + kLineAhead, f);
+                load(m + kRowsAhead, f);
             }
             wave_sync();
             vstore(x2, m0);
@@ -426,18 +584,18 @@ k_line_solve(int M, int N, int S, const double* __restrict__ F, const double* r,
             }
         };
         const int mt = (M - 1) / kLineBlock * kLineBlock;   // first step of the top block
-        // step m = m0 + kLineBlock - 1 - k of a block uses ring slot m % kLineAhead
-        LineBwd ring[kLineAhead];
+        // step m = m0 + kLineBlock - 1 - k of a block uses ring slot m % kRowsAhead
+        LineBwd ring[kRowsAhead];
 #pragma unroll
-        for (int k = 0; k < kLineAhead; ++k)
-            load(mt + kLineBlock - 1 - k, ring[(kLineBlock - 1 - k) % kLineAhead]);
+        for (int k = 0; k < kRowsAhead; ++k)
+            load(mt + kLineBlock - 1 - k, ring[(kLineBlock - 1 - k) % kRowsAhead]);
         for (int m0 = mt; m0 >= 0; m0 -= kLineBlock) {
             vload(x2, m0);
             wave_sync();
 #pragma unroll
             for (int k = 0; k < kLineBlock; ++k) {
                 const int q = kLineBlock - 1 - k, m = m0 + q;
-                LineBwd& b = ring[q % kLineAhead];
+                LineBwd& b = ring[q % kRowsAhead];
                 if (m < M) {
                     const double2 yv = buf[lane][q];
                     Z acc = {yv.x, yv.y};
@@ -461,7 +619,7 @@ k_line_solve(int M, int N, int S, const double* __restrict__ F, const double* r,
                         x1[j] = xq[j];
                     }
                 }
-                load(m - kLineAhead, b);
+                load(m - kRowsAhead, b);
             }
             wave_sync();
             vstore(x2, m0);
@@ -1085,6 +1243,12 @@ int pea_line_factor(int M, int N, int S, const double* JU, const double* shifts,
     return hipGetLastError() == hipSuccess ? PEA_OK : rfail(PEA_ERR_HIP, "k_line_factor");
 }
 
+int pea_line_solve_lanes(int lanes) {
+    if (lanes != 0 && lanes != 4 && lanes != 16) return rfail(PEA_ERR_DIMS, "lanes must be 0, 4 or 16");
+    g_line_lanes = lanes;
+    return PEA_OK;
+}
+
 int pea_line_solve(int M, int N, int S, const double* JU, const double* F, int transposed,
                    const double* r, double* x, void* stream) {
     if (M < 4 || N < 5 || S < 1 || S > 65535) return rfail(PEA_ERR_DIMS, "bad dimensions");
@@ -1094,8 +1258,16 @@ int pea_line_solve(int M, int N, int S, const double* JU, const double* F, int t
     // the factorisation stored the system's blocks (transposed or not): the solve is the same
     (void)transposed;
     (void)JU;
-    k_line_solve<<<dim3((4 * N + kLineThreads - 1) / kLineThreads, S), kLineThreads, 0,
-                   (hipStream_t)stream>>>(M, N, S, F, r, x);
+    // four lanes per line while that still gives two waves per SIMD (S*N/16 >= 2048), else
+    // sixteen: the rows form moves fewer values between lanes, the elements form has 4x the
+    // parallelism for few shifts
+    const int lanes = g_line_lanes ? g_line_lanes : ((long)S * N >= 32768 ? 4 : 16);
+    if (lanes == 4)
+        k_line_solve_rows<<<dim3((4 * N + kLineThreads - 1) / kLineThreads, S), kLineThreads, 0,
+                             (hipStream_t)stream>>>(M, N, S, F, r, x);
+    else
+        k_line_solve_elems<<<dim3((16 * N + kLineThreads - 1) / kLineThreads, S), kLineThreads, 0,
+                              (hipStream_t)stream>>>(M, N, S, F, r, x);
     return hipGetLastError() == hipSuccess ? PEA_OK : rfail(PEA_ERR_HIP, "k_line_solve");
 }
 

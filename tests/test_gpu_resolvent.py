@@ -63,9 +63,10 @@ def sd_ragged():
     return s
 
 
+@pytest.mark.parametrize("lanes", [4, 16])
 @pytest.mark.parametrize("grid", ["33x17", "43x22"])
 @pytest.mark.parametrize("adjoint", [False, True])
-def test_line_preconditioner_solves_each_radial_line(request, grid, adjoint):
+def test_line_preconditioner_solves_each_radial_line(request, grid, adjoint, lanes):
     """pea_line_factor/pea_line_solve against dense solves of every radial line's system in numpy:
     the line's blocks ii, i, c, o, oo of dR/dU minus s I (transposed, minus conj(s) I, for the
     adjoint), for several shifts at once."""
@@ -73,7 +74,7 @@ def test_line_preconditioner_solves_each_radial_line(request, grid, adjoint):
     from pyeulerale_amd import kernels as K
     from pyeulerale_amd.gmres import _p
     sd = request.getfixturevalue("sd" if grid == "33x17" else "sd_ragged")
-    print(f"[resolvent] line preconditioner {grid} adjoint={adjoint}", flush=True)
+    print(f"[resolvent] line preconditioner {grid} adjoint={adjoint} lanes={lanes}", flush=True)
     g = sd._grid
     M, N = g.M, g.N
     JU = sd._get_blocks()["U"]
@@ -102,12 +103,17 @@ def test_line_preconditioner_solves_each_radial_line(request, grid, adjoint):
     _lib.call("pea_line_factor", M, N, S, _p(JU), _p(sh_d), int(adjoint), _p(F), K._stream())
     bd = torch.as_tensor(b).cuda()
     x = torch.empty_like(bd)
-    _lib.call("pea_line_solve", M, N, S, _p(JU), _p(F), int(adjoint), _p(bd), _p(x), K._stream())
-    assert rel(x.cpu().numpy(), ref) <= 1e-11
-    # in place (x aliasing r): the same result
-    _lib.call("pea_line_solve", M, N, S, _p(JU), _p(F), int(adjoint), _p(bd), _p(bd),
-              K._stream())
-    assert torch.equal(bd, x)
+    _lib.call("pea_line_solve_lanes", lanes)
+    try:
+        _lib.call("pea_line_solve", M, N, S, _p(JU), _p(F), int(adjoint), _p(bd), _p(x),
+                  K._stream())
+        assert rel(x.cpu().numpy(), ref) <= 1e-11
+        # in place (x aliasing r): the same result
+        _lib.call("pea_line_solve", M, N, S, _p(JU), _p(F), int(adjoint), _p(bd), _p(bd),
+                  K._stream())
+        assert torch.equal(bd, x)
+    finally:
+        _lib.call("pea_line_solve_lanes", 0)
 
 
 @pytest.mark.parametrize("q", [1, 5, 16, 23, 40])
