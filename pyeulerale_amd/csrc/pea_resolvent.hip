@@ -242,43 +242,64 @@ __device__ __forceinline__ B4 load_b(const double* F, int M, int N, int sh, int 
     return b;
 }
 
-__global__ void __launch_bounds__(64)
+// The blocks D^-1 and U1 of the two previous steps stay in LDS, in a ring of two per kind,
+// [kind][element][lane] so that each element's 64 lanes are contiguous: reading them back from
+// F would wait on this thread's own recent stores, 256 B apart across lanes.
+constexpr int kFactorThreads = 64;
+
+__global__ void __launch_bounds__(kFactorThreads)
 k_line_factor(int M, int N, int S, const double* __restrict__ sh, int tr, double* __restrict__ F) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ Z ring[4][16][kFactorThreads];   // kinds: Dinv even/odd step, U1 even/odd step
+    const int lane = threadIdx.x;
+    const int n = blockIdx.x * kFactorThreads + lane;
     const int s = blockIdx.y;
-    if (n >= N || s >= S) return;
+    if (n >= N || s >= S) return;   // per thread: each lane touches only its own LDS column
     const long C = (long)M * N;
     const double* SB = F + sb_offset(M, N, S);
+    auto rput = [&](int kind, const B4& b) {
+#pragma unroll
+        for (int ij = 0; ij < 16; ++ij) ring[kind][ij][lane] = b.a[ij / 4][ij % 4];
+    };
+    auto rget = [&](int kind) {
+        B4 b;
+#pragma unroll
+        for (int ij = 0; ij < 16; ++ij) b.a[ij / 4][ij % 4] = ring[kind][ij][lane];
+        return b;
+    };
     // adjoint: (J - s I)^H = J^T - conj(s) I
     const Z shift = tr ? Z{sh[2 * s], -sh[2 * s + 1]} : Z{sh[2 * s], sh[2 * s + 1]};
     const Z zero = {0.0, 0.0};
     for (int m = 0; m < M; ++m) {
+        const int cur = m & 1, prev = cur ^ 1;   // ring slot of steps m (and m - 2), m - 1
         B4 D = sb_block(SB, C, N, n, m, 0, shift);
         B4 L2, L1;
         if (m >= 2) {
-            L2 = bmul(sb_block(SB, C, N, n, m, -2, zero), load_b(F, M, N, s, m - 2, 2, n));
+            L2 = bmul(sb_block(SB, C, N, n, m, -2, zero), rget(cur));
             // D -= L2 * U2(m-2),  U2(m-2) = block (m-2, m)
             D = bsub(D, bmul(L2, sb_block(SB, C, N, n, m - 2, 2, zero)));
             store_b(F, M, N, s, m, 0, n, L2);
         }
         if (m >= 1) {
             B4 B1 = sb_block(SB, C, N, n, m, -1, zero);
-            if (m >= 2) B1 = bsub(B1, bmul(L2, load_b(F, M, N, s, m - 2, 3, n)));
-            L1 = bmul(B1, load_b(F, M, N, s, m - 1, 2, n));
-            D = bsub(D, bmul(L1, load_b(F, M, N, s, m - 1, 3, n)));
+            if (m >= 2) B1 = bsub(B1, bmul(L2, rget(2 + cur)));
+            L1 = bmul(B1, rget(prev));
+            D = bsub(D, bmul(L1, rget(2 + prev)));
             store_b(F, M, N, s, m, 1, n, L1);
         }
         if (m + 1 < M) {
             B4 U1 = sb_block(SB, C, N, n, m, 1, zero);
             if (m >= 1) U1 = bsub(U1, bmul(L1, sb_block(SB, C, N, n, m - 1, 2, zero)));
             store_b(F, M, N, s, m, 3, n, U1);
+            rput(2 + cur, U1);
         }
         if (s == 0 && m + 2 < M) {
             const B4 U2 = sb_block(SB, C, N, n, m, 2, zero);
 #pragma unroll
             for (int ij = 0; ij < 16; ++ij) F[u2idx(M, N, S, m, ij, n)] = U2.a[ij / 4][ij % 4].r;
         }
-        store_b(F, M, N, s, m, 2, n, binv(D));
+        const B4 Di = binv(D);
+        store_b(F, M, N, s, m, 2, n, Di);
+        rput(cur, Di);
     }
 }
 
@@ -1238,8 +1259,8 @@ int pea_line_factor(int M, int N, int S, const double* JU, const double* shifts,
     if (!JU || !shifts || !F) return rfail(PEA_ERR_NULL, "null argument");
     k_line_blocks<<<dim3((M + kTrM - 1) / kTrM, (N + kTrN - 1) / kTrN, 80), 256, 0,
                     (hipStream_t)stream>>>(M, N, JU, transposed, F + sb_offset(M, N, S));
-    k_line_factor<<<dim3((N + 63) / 64, S), 64, 0, (hipStream_t)stream>>>(M, N, S, shifts,
-                                                                         transposed, F);
+    k_line_factor<<<dim3((N + kFactorThreads - 1) / kFactorThreads, S), kFactorThreads, 0,
+                    (hipStream_t)stream>>>(M, N, S, shifts, transposed, F);
     return hipGetLastError() == hipSuccess ? PEA_OK : rfail(PEA_ERR_HIP, "k_line_factor");
 }
 
