@@ -261,3 +261,44 @@ def test_hvp_against_central_differences_of_vjp(name, K):
     for k in want:
         got = H[k].cpu().numpy() if k == "mach" else host(H[k])
         assert rel(got, fd[k]) <= 1e-6, (k, rel(got, fd[k]))
+
+
+def test_coefficients_and_pressure_tangent():
+    """The coefficient operators (core.py:1277-1488) through the identities their definitions fix.
+    c_l, c_m and c_d are linear in the forces, so grad . F reproduces them, except that c_d's
+    gradient carries the 1/L its value lacks (the reference's quirk, core.py:1286 and 1289-1311).
+    With the forces held, c_m is linear in the surface vertices, so its vertex gradient predicts a
+    vertex move exactly.  The c_p tangent matches central differences of c_p."""
+    from pyeulerale_amd.core import SpatialDiscretization
+    X, V, U, mach, aoa = setup_case("small_pitch")
+    L = 2.5
+    sd = SpatialDiscretization(vertices=X, angle_of_attack=aoa, mach_number=mach,
+                               coefficient_length=L, coefficient_center=(0.25, 0.05))
+    sd.states[:] = torch.as_tensor(U).cuda()
+    sd.compute_forces()
+    F = sd.forces.to(torch.complex128)
+    cl, cd, cm = (sd.compute_lift_coefficient(), sd.compute_drag_coefficient(),
+                  sd.compute_moment_coefficient())
+    def check(grad, value):   # grad . F == value, to round-off of the summed terms
+        terms = grad * F
+        assert abs(float(terms.sum().real) - value) <= 1e-13 * float(terms.abs().sum())
+    check(sd.compute_lift_coefficient_wrt_forces(), cl)
+    check(sd.compute_moment_coefficient_wrt_forces(), cm)
+    check(sd.compute_drag_coefficient_wrt_forces(), cd / L)
+    rng = np.random.default_rng(3)
+    dX = np.zeros_like(X)
+    dX[:, 0, :] = 1e-3 * rng.standard_normal(dX[:, 0, :].shape)
+    dX[:, 0, -1] = dX[:, 0, 0]                     # the seam vertex moves with its twin
+    pred = (sd.compute_moment_coefficient_wrt_vertices() * torch.as_tensor(dX).cuda()).sum().real
+    sd.vertices[:] = torch.as_tensor(X + dX).cuda()
+    moved = sd.compute_moment_coefficient()        # forces held: not recomputed
+    assert abs(moved - cm - float(pred)) <= 1e-12 * abs(float(pred))
+    sd.vertices[:] = torch.as_tensor(X).cuda()
+    dU = rng.standard_normal(U.shape)
+    h = 1e-6
+    cp0 = lambda W: (sd.states.copy_(torch.as_tensor(W).cuda()),
+                     sd.compute_pressure_coefficients().clone())[1]
+    fd = (cp0(U + h * dU) - cp0(U - h * dU)) / (2 * h)
+    sd.states[:] = torch.as_tensor(U).cuda()
+    tan = sd.apply_pressure_coefficients_wrt_states_fwd(torch.as_tensor(dU).cuda())
+    assert rel(tan.real.cpu().numpy(), fd.cpu().numpy()) <= 1e-8
