@@ -185,6 +185,12 @@ int pea_cs_jvp_odes(const pea_params *prm, double mach, int M, int N, const doub
                     const double *V, const double *U, const double *dmach, const double *dX,
                     const double *dV, const double *dU, double h, double *dR, void *stream);
 
+/* k states directions at once (dU: k consecutive complex arrays; dR likewise): the directional
+ * complex step, one direction per grid y, as pea_cs_jvp_odes per direction. */
+int pea_cs_jvp_odes_multi(const pea_params *prm, double mach, int M, int N, int k,
+                          const double *X, const double *V, const double *U, const double *dU,
+                          double h, double *dR, void *stream);
+
 /* Complex-step Jacobian assembly at (Ma, X, V, U), the reference's linearize (core.py:278-292):
  *   JU  9 blocks/cell  JU[((s*4 + i)*4 + j)*C + n*M + m]  = dR_i/dU_j of neighbour s
  *       (s = c,b,bb,f,ff,i,ii,o,oo as f90:699-707; absent radial neighbours are written 0)

@@ -51,6 +51,7 @@ _SIGS = {
     "pea_hvp_forces": [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P],
     "pea_hvp_odes": [_P, _D, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "pea_cs_jvp_odes": [_P, _D, _I, _I, _P, _P, _P, _P, _P, _P, _P, _D, _P, _P],
+    "pea_cs_jvp_odes_multi": [_P, _D, _I, _I, _I, _P, _P, _P, _P, _D, _P, _P],
     "pea_cs_linearize_odes": [_P, _D, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P],
     "pea_apply_odes_wrt_states_fwd": [_I, _I, _P, _P, _P, _P],
     "pea_apply_odes_wrt_states_rev": [_I, _I, _P, _P, _P, _P],

@@ -258,6 +258,12 @@ struct LdCxU {
         w[1] = buf_load(r, p, g.P);
     }
     PEA_HD double machv() const { return mach; }
+    // direction b of a batch launch (grid y)
+    PEA_HD LdCxU at(int b) const {
+        LdCxU r = *this;
+        r.dU += (long)b * 8 * g.C;
+        return r;
+    }
 };
 
 // Complex step along part `part` (0 real, 1 imaginary) of complex directions:
@@ -285,6 +291,7 @@ struct LdCx {
     PEA_HD void vertex(int p, Cx* x) const { vec2(X, dX, p, x); }
     PEA_HD void velocity(int p, Cx* w) const { vec2(V, dV, p, w); }
     PEA_HD Cx machv() const { return Cx{mach, dmach ? h * dmach[part] : 0.0}; }
+    PEA_HD LdCx at(int) const { return *this; }   // not launched as a batch
 };
 
 // Stencil source of one cell over a loader.
@@ -697,12 +704,13 @@ k_cs_jvp(pea_params P, Ld ld, double* __restrict__ dR) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ld.g.C) return;
     using G = decltype(ld.machv());
-    Ld l0 = ld;
+    double* out = dR + (long)blockIdx.y * 8 * ld.g.C;   // direction y of a batch
+    Ld l0 = ld.at(blockIdx.y);
     l0.part = 0;
-    residual_cell<Cx, G>(P, l0, c, StoreCs{dR, ld.h, ld.g.C, 0});
-    Ld l1 = ld;
+    residual_cell<Cx, G>(P, l0, c, StoreCs{out, ld.h, ld.g.C, 0});
+    Ld l1 = ld.at(blockIdx.y);
     l1.part = 1;
-    residual_cell<Cx, G>(P, l1, c, StoreCs{dR, ld.h, ld.g.C, 1});
+    residual_cell<Cx, G>(P, l1, c, StoreCs{out, ld.h, ld.g.C, 1});
 }
 
 // Complex-step block assembly (the reference's linearize): one thread per cell, one complex
@@ -2211,6 +2219,21 @@ int pea_cs_jvp_odes(const pea_params* prm, double mach, int M, int N, const doub
     else
         k_cs_jvp<<<blocks_for(g.C, bs), bs, 0, as_stream(stream)>>>(
             *prm, LdCx{X, V, U, dX, dV, dU, dmach, mach, h, 0, g}, dR);
+    PEA_LAUNCH_CHECK("k_cs_jvp");
+    return PEA_OK;
+}
+
+int pea_cs_jvp_odes_multi(const pea_params* prm, double mach, int M, int N, int k,
+                          const double* X, const double* V, const double* U, const double* dU,
+                          double h, double* dR, void* stream) {
+    if (int rc = check_dims(M, N)) return rc;
+    if (!prm || !X || !V || !U || !dU || !dR) return fail(PEA_ERR_NULL, "null argument");
+    if (k < 1 || k > 65535) return fail(PEA_ERR_ARG, "k must be 1..65535 (got %d)", k);
+    if (!(h > 0.0)) h = kCsStep;
+    const Grid g = make_grid(M, N);
+    const int bs = cell_block(g.C);
+    k_cs_jvp<<<dim3(blocks_for(g.C, bs), k), bs, 0, as_stream(stream)>>>(
+        *prm, LdCxU{X, V, U, dU, mach, h, 0, g}, dR);
     PEA_LAUNCH_CHECK("k_cs_jvp");
     return PEA_OK;
 }

@@ -411,6 +411,18 @@ def cs_jvp_odes(p: Params, mach: float, g: Grid, X, V, U, dmach=None, dX=None, d
     return dR
 
 
+def cs_jvp_odes_multi(p: Params, mach: float, g: Grid, X, V, U, dU, h: float = 0.0,
+                      dR=None) -> torch.Tensor:
+    """Directional complex step along k states directions at once (dU of shape (k, 4, N, M))."""
+    g.check_primal(X, V, U)
+    k = dU.shape[0]
+    _need(dU, (k, *g.cell_shape), C128, "d_states")
+    dR = torch.empty_like(dU) if dR is None else _need(dR, (k, *g.cell_shape), C128, "d_odes")
+    _lib.call("pea_cs_jvp_odes_multi", ctypes.byref(p.c()), float(mach), g.M, g.N, k, _ptr(X),
+              _ptr(V), _ptr(U), _ptr(dU), float(h), _ptr(dR), _stream())
+    return dR
+
+
 def cs_linearize_odes(p: Params, mach: float, g: Grid, X, V, U, JU=None, JX=None, JV=None,
                       JMa=None, want=("states", "vertices", "velocities", "mach")):
     """Complex-step Jacobian blocks (the reference's linearize).  Storage:

@@ -167,6 +167,10 @@ def test_jvp_multi_directions(name, orc, K):
     for i in range(2):
         one = K.jvp_odes(p, mach, g, Xd, Vd, Ud, dU=dUd[i].contiguous(), dX=dXs[i].contiguous())
         assert rel(host(out2[i]), host(one)) <= 1e-13
+    # the directional complex step, k directions per launch, equals one launch per direction
+    cs = K.cs_jvp_odes_multi(p, mach, g, Xd, Vd, Ud, dUd)
+    for i in range(k):
+        assert torch.equal(cs[i], K.cs_jvp_odes(p, mach, g, Xd, Vd, Ud, dU=dUd[i].contiguous()))
     # k seeds of the states adjoint: one launch of the face adjoint for N >= 18, one adjoint
     # per seed below (tiny: N = 8)
     gs = K.vjp_odes_states_multi(p, mach, g, Xd, Vd, Ud, dUd)
