@@ -1,15 +1,16 @@
 // Kernels of the resolvent (dR/dU - s I)^-1 for many shifts s at once.
 //
-// * k_block_fwd_multi / k_block_rev_multi: products of the assembled dR/dU blocks with S complex
+// * k_block_multi: products of the assembled dR/dU blocks (or their transposes) with S complex
 //   vectors (operator of batched GMRES).  A workgroup stages the 9 blocks of 32 cells in LDS once
 //   and applies them to all vectors, so the 1152 B/cell of blocks are read once per product
 //   batch instead of once per vector.  The shift is applied in the same pass: y = J x - s x.
-// * k_line_factor / k_line_solve: radial-line preconditioner.  For every angular index n and
-//   shift s the block-pentadiagonal matrix of the radial line (blocks ii, i, c - sI, o, oo) is
-//   factorised by block LU (partial pivoting inside each 4x4 diagonal block) and solved.
-//   Factorisation: one thread per (line, shift), m sequential; solve: sixteen lanes per (line,
-//   shift), one per element of the 4x4 blocks.  The transposed variant factorises the line of
-//   (dR/dU)^T - conj(s) I for the adjoint solve.
+// * k_line_blocks / k_line_factor / k_line_solve_rows / k_line_solve_elems: radial-line
+//   preconditioner.  For every angular index n and shift s the block-pentadiagonal matrix of the
+//   radial line (blocks ii, i, c - sI, o, oo) is factorised by block LU (partial pivoting inside
+//   each 4x4 diagonal block) and solved.  k_line_blocks transposes the line blocks lines-fastest;
+//   the factorisation runs one thread per (line, shift), m sequential; the solve four lanes per
+//   (line, shift) (one per block row) or sixteen (one per block element).  The transposed variant
+//   factorises the line of (dR/dU)^T - conj(s) I for the adjoint solve.
 // * k_jacobi_factor / k_jacobi_solve: block-Jacobi preconditioner (4x4 diagonal blocks).
 // * k_ring_band / k_ring_couple: pieces of the exact block-banded LU over radial rings.  A ring
 //   is every cell at one radial index m; ordered (n, v) its unknowns form a dense 4N x 4N
@@ -18,9 +19,12 @@
 //   block diagonal in n (blocks ii, i, o, oo).  k_ring_band writes the dense ring block for all
 //   shifts; k_ring_couple forms Z = beta Z + alpha C Y with a coupling C, or C itself as a dense
 //   matrix when Y is null.  The dense eliminations between them are library GEMM/LU calls.
-// * k_multidot_partial / k_gs_update / k_multidot_final: Gram-Schmidt of batched GMRES.  A
-//   projection pass reads the basis once; the update pass subtracts the projections and, reading
-//   the basis rows only once more, forms the next projections or the norm of the result.
+// * k_multidot_partial / k_dgks_update / k_gs_update / k_multidot_final / k_dgks_finish:
+//   Gram-Schmidt of batched GMRES.  A projection pass reads the basis once; an update pass
+//   subtracts the projections and, reading the basis rows only once more, forms the next
+//   projections or the norm of the result; a second update runs only for the shifts whose
+//   first projection removed more than half of the vector's norm squared.
+// * k_gmres_rotate: Givens updates of the Hessenberg matrices, one thread per shift.
 #include <hip/hip_runtime.h>
 
 #include <rocblas/rocblas.h>
