@@ -125,3 +125,17 @@ def test_example_fixtures_parse_and_grids_read():
     assert freq[0][1] == complex("+7.386e+00-3.482e-01j")
     ptc, freq, gains = parse_out(gold / "gain_Ma0p50.out")
     assert len(ptc) == 11 and not freq and gains == [8.2518, 7.1469]
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus N > 1 exits with status 2 and says so in its JSON line."""
+    import json
+    import subprocess
+    import sys
+    from pathlib import Path
+    root = Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    line = json.loads(r.stdout.strip().splitlines()[-1])
+    assert line["n_gpus"] == 2 and "one GPU" in line["error"]
