@@ -61,8 +61,12 @@ def parse():
                     help="skip timing the reference's host SuperLU solve")
     ap.add_argument("--profile", action="store_true",
                     help="plain launches only (no graphs) for counter collection: a 512 MiB "
-                         "copy for calibration, the batched residual, and the JVP / VJP / "
-                         "complex-step products at 1025x513, 5 launches each")
+                         "copy for calibration, the batched residual, the JVP / VJP / "
+                         "complex-step products at 1025x513 and the GMRES line solve, 5 "
+                         "launches each")
+    ap.add_argument("--only-gmres", action="store_true",
+                    help="only the 1025x513 batched GMRES section (its JSON on one line), for "
+                         "a kernel trace of the Arnoldi steps")
     return ap.parse_args()
 
 
@@ -567,6 +571,9 @@ def main():
     if args.profile:
         profile_launches(torch, K)
         progress("profile launches done")
+        return
+    if args.only_gmres:
+        print(json.dumps(gmres_steps(torch, K)), flush=True)
         return
     mach, aoa = 0.6, 0.0
     omega, t_ = 0.2, 2.0
