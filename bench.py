@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--only-gmres", action="store_true",
                     help="only the 1025x513 batched GMRES section (its JSON on one line), for "
                          "a kernel trace of the Arnoldi steps")
+    ap.add_argument("--only-multigrid", action="store_true",
+                    help="only the 64-grid section (its JSON on one line), for a kernel trace")
     return ap.parse_args()
 
 
@@ -574,6 +576,9 @@ def main():
         return
     if args.only_gmres:
         print(json.dumps(gmres_steps(torch, K)), flush=True)
+        return
+    if args.only_multigrid:
+        print(json.dumps(multigrid(torch, K)), flush=True)
         return
     mach, aoa = 0.6, 0.0
     omega, t_ = 0.2, 2.0
