@@ -1,7 +1,7 @@
 // Kernels of the resolvent (dR/dU - s I)^-1 for many shifts s at once.
 //
 // * k_block_multi: products of the assembled dR/dU blocks (or their transposes) with S complex
-//   vectors (operator of batched GMRES).  A workgroup stages the 9 blocks of 32 cells in LDS once
+//   vectors (operator of batched GMRES).  A workgroup stages the 9 blocks of 64 cells in LDS once
 //   and applies them to all vectors, so the 1152 B/cell of blocks are read once per product
 //   batch instead of once per vector.  The shift is applied in the same pass: y = J x - s x.
 // * k_line_blocks / k_line_factor / k_line_solve_rows / k_line_solve_elems: radial-line
@@ -46,6 +46,10 @@ __device__ __forceinline__ Z zsub(Z a, Z b) { return {a.r - b.r, a.i - b.i}; }
 __device__ __forceinline__ Z zmul(Z a, Z b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
 __device__ __forceinline__ Z zfma(Z a, Z b, Z c) {   // a*b + c
     return {fma(a.r, b.r, fma(-a.i, b.i, c.r)), fma(a.r, b.i, fma(a.i, b.r, c.i))};
+}
+// a*b + c for real a: two FMAs (zfma with a zero imaginary part would keep the 0*b terms)
+__device__ __forceinline__ Z zfma_re(double a, Z b, Z c) {
+    return {fma(a, b.r, c.r), fma(a, b.i, c.i)};
 }
 __device__ __forceinline__ Z zinv(Z a) {
     const double d = a.r * a.r + a.i * a.i;
@@ -699,84 +703,101 @@ k_jacobi_solve(int C, int S, const double* __restrict__ F, const double* r, doub
     }
 }
 
-// y_k = J x_k - s_k x_k (or J^T x_k - conj(s_k) x_k) for k < S.  A workgroup holds 32 cells'
-// nine blocks in LDS and loops over the vectors; thread (cell, k-slot) handles vectors
-// k-slot, k-slot + 8, ...
-constexpr int kMvCells = 32, kMvSlots = 8;
+// y_k = J x_k - s_k x_k (or J^T x_k - conj(s_k) x_k) for k < S.  A workgroup holds 64 cells'
+// nine blocks in LDS; wave w handles vectors w, w + 4, ..., one cell per lane.  Blocks of
+// neighbours outside the grid (radially) are staged as zeros and read the cell's own vector
+// instead, so both loops are straight-line code whose loads all issue before the first wait
+// (fma(0, x, a) = a, so the sums are unchanged).
+constexpr int kMvCells = 64, kMvSlots = 4, kMvPasses = 16 / kMvSlots;
+// slot s of a cell's nine blocks: the cell itself, angular neighbours n-1, n-2, n+1, n+2
+// (periodic), radial neighbours m-1, m-2, m+1, m+2
+__device__ constexpr int kSlotDm[9] = {0, 0, 0, 0, 0, -1, -2, 1, 2};
+__device__ constexpr int kSlotDn[9] = {0, -1, -2, 1, 2, 0, 0, 0, 0};
 
-__global__ void __launch_bounds__(kMvCells * kMvSlots)
+// two workgroups per CU (LDS 2 x 72 KB): the register allocation must fit two waves per SIMD
+__global__ void __launch_bounds__(kMvCells * kMvSlots) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_block_multi(int M, int N, int S, const double* __restrict__ JU, const double* __restrict__ sh,
               int tr, const double* __restrict__ x, double* __restrict__ y) {
+    static_assert(kMvCells == 64 && kMvSlots * kMvPasses == 16, "one wave per vector slot");
     __shared__ double sJ[9][16][kMvCells];
     const int C = M * N;
     const int lc = threadIdx.x % kMvCells, slot = threadIdx.x / kMvCells;
     const int c0 = blockIdx.x * kMvCells;
-    // forward: blocks of this cell; transposed: block s of the neighbour k = c - offset(s)
-    for (int i = threadIdx.x; i < 9 * 16 * kMvCells; i += kMvCells * kMvSlots) {
-        const int cc = i % kMvCells, e = (i / kMvCells) % 16, s = i / (16 * kMvCells);
-        const int c = c0 + cc;
-        double v = 0.0;
-        if (c < C) {
-            if (!tr) {
-                v = JU[(long)(s * 16 + e) * C + c];
-            } else {
-                const int n = c / M, m = c - n * M;
-                const int dm[9] = {0, 0, 0, 0, 0, -1, -2, 1, 2};
-                const int dn[9] = {0, -1, -2, 1, 2, 0, 0, 0, 0};
-                const int mk = m - dm[s];
-                if (mk >= 0 && mk < M) {
-                    int nk = n - dn[s];
-                    nk = nk < 0 ? nk + N : (nk >= N ? nk - N : nk);
-                    // transposed block: element (i, j) of J^T block = (j, i) of J block
-                    const int ii = e / 4, jj = e % 4;
-                    v = JU[(long)(s * 16 + jj * 4 + ii) * C + nk * M + mk];
-                }
+    auto wr = [&](int q) { return q < 0 ? q + N : (q >= N ? q - N : q); };
+    {
+        // forward: blocks of this cell; transposed: block s of the neighbour k = c - offset(s),
+        // element (i, j) of its J^T block = (j, i) of the J block
+        const int c = c0 + lc;
+        const int cs = c < C ? c : 0;
+        const int n = cs / M, m = cs - n * M;
+        double v[9][kMvPasses];
+#pragma unroll
+        for (int s = 0; s < 9; ++s) {
+            const int mk = tr ? m - kSlotDm[s] : m + kSlotDm[s];
+            const bool ok = c < C && mk >= 0 && mk < M;
+            const int k = tr ? (ok ? wr(n - kSlotDn[s]) * M + mk : cs) : cs;
+#pragma unroll
+            for (int h = 0; h < kMvPasses; ++h) {
+                const int e = h * kMvSlots + slot;
+                const int ee = tr ? (e % 4) * 4 + e / 4 : e;
+                const double t = JU[(long)(s * 16 + ee) * C + k];
+                v[s][h] = ok ? t : 0.0;
             }
         }
-        sJ[s][e][cc] = v;
+#pragma unroll
+        for (int s = 0; s < 9; ++s)
+#pragma unroll
+            for (int h = 0; h < kMvPasses; ++h) sJ[s][h * kMvSlots + slot][lc] = v[s][h];
     }
     __syncthreads();
     const int c = c0 + lc;
     if (c >= C) return;
     const int n = c / M, m = c - n * M;
     int nb[9];
-    {
-        auto wr = [&](int q) { return q < 0 ? q + N : (q >= N ? q - N : q); };
-        // forward: neighbour s of c;  transposed: cell k = c - offset(s)  (offset of slot s)
-        const int dm[9] = {0, 0, 0, 0, 0, -1, -2, 1, 2};
-        const int dn[9] = {0, -1, -2, 1, 2, 0, 0, 0, 0};
+    // forward: neighbour s of c;  transposed: cell k = c - offset(s); the cell itself where
+    // that lies outside the grid (its staged block is zero)
 #pragma unroll
-        for (int s = 0; s < 9; ++s) {
-            const int mm = tr ? m - dm[s] : m + dm[s];
-            const int nn = wr(tr ? n - dn[s] : n + dn[s]);
-            nb[s] = (mm >= 0 && mm < M) ? nn * M + mm : -1;
-        }
+    for (int s = 0; s < 9; ++s) {
+        const int mm = tr ? m - kSlotDm[s] : m + kSlotDm[s];
+        const int nn = wr(tr ? n - kSlotDn[s] : n + kSlotDn[s]);
+        nb[s] = (mm >= 0 && mm < M) ? nn * M + mm : c;
     }
     for (int k = slot; k < S; k += kMvSlots) {
-        const long base = (long)k * 4 * C;
-        Z acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+        // keeps the block reads from LDS inside the loop: hoisted, they would hold 288 registers
+        asm volatile("" ::: "memory");
+        // one buffer descriptor per vector (k is the wave's), 32-bit lane offsets (cells < 2^24)
+        // and the component as a scalar offset: no 64-bit address is formed per load
+        const int kw = __builtin_amdgcn_readfirstlane(k);
+        const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(x + (long)kw * 8 * C), (short)0, 64 * C, 0x00020000);
+        Z v[9][4];
 #pragma unroll
-        for (int s = 0; s < 9; ++s) {
-            if (nb[s] < 0) continue;
-            Z v[4];
+        for (int s = 0; s < 9; ++s)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const long e = base + (long)j * C + nb[s];
-                v[j] = Z{x[2 * e], x[2 * e + 1]};
+                const double2 t = __builtin_bit_cast(
+                    double2, __builtin_amdgcn_raw_buffer_load_b128(xr, nb[s] * 16, j * C * 16, 0));
+                v[s][j] = Z{t.x, t.y};
             }
+        // all 36 loads in flight before the first wait (the scheduler would otherwise sink
+        // them to their uses, two at a time)
+        __builtin_amdgcn_sched_barrier(0);
+        Z acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+#pragma unroll
+        for (int s = 0; s < 9; ++s)
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i] = zfma(Z{sJ[s][4 * i + j][lc], 0.0}, v[j], acc[i]);
-        }
+                for (int j = 0; j < 4; ++j)
+                    acc[i] = zfma_re(sJ[s][4 * i + j][lc], v[s][j], acc[i]);
         const Z shift = tr ? Z{sh[2 * k], -sh[2 * k + 1]} : Z{sh[2 * k], sh[2 * k + 1]};
+        double* yk = y + (long)k * 8 * C;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const long e = base + (long)i * C + c;
-            const Z xi = Z{x[2 * e], x[2 * e + 1]};
-            const Z r = zsub(acc[i], zmul(shift, xi));
-            y[2 * e] = r.r;
-            y[2 * e + 1] = r.i;
+            const Z r = zsub(acc[i], zmul(shift, v[0][i]));
+            const long e = (long)i * C + c;
+            yk[2 * e] = r.r;
+            yk[2 * e + 1] = r.i;
         }
     }
 }
@@ -1320,6 +1341,7 @@ int pea_block_multi(int M, int N, int S, const double* JU, const double* shifts,
     if (M < 4 || N < 5 || S < 1) return rfail(PEA_ERR_DIMS, "bad dimensions");
     if (!JU || !shifts || !x || !y) return rfail(PEA_ERR_NULL, "null argument");
     const int C = M * N;
+    if (C >= (1 << 24)) return rfail(PEA_ERR_DIMS, "block product: at most 2^24 - 1 cells");
     k_block_multi<<<(C + kMvCells - 1) / kMvCells, kMvCells * kMvSlots, 0, (hipStream_t)stream>>>(
         M, N, S, JU, shifts, transposed, x, y);
     return hipGetLastError() == hipSuccess ? PEA_OK : rfail(PEA_ERR_HIP, "k_block_multi");

@@ -27,14 +27,18 @@ def sd():
     return s
 
 
-def test_block_multi_matches_single_products(sd):
+@pytest.mark.parametrize("S", [5, 11])
+@pytest.mark.parametrize("grid", ["33x17", "43x22"])
+def test_block_multi_matches_single_products(request, grid, S):
+    """S = 5 leaves a wave's second slot idle; S = 11 makes slots loop over a second vector;
+    43x22 has a cell count that is not a multiple of the kernel's 32-cell workgroups."""
     from pyeulerale_amd import _lib
     from pyeulerale_amd import kernels as K
     from pyeulerale_amd.gmres import _p
+    sd = request.getfixturevalue("sd" if grid == "33x17" else "sd_ragged")
     g = sd._grid
     JU = sd._get_blocks()["U"]
-    rng = np.random.default_rng(0)
-    S = 5
+    rng = np.random.default_rng(S)
     x = torch.as_tensor(crandn(rng, (S, *g.cell_shape))).cuda()
     shifts = torch.as_tensor(crandn(rng, S)).cuda()
     for tr in (0, 1):
