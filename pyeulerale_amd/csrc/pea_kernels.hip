@@ -459,14 +459,8 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const FsDir& 
 
     auto vtx = [&](int mm, int nn, G* x) { l.vertex(nn * (g.M + 1) + mm, x); };
     auto vel = [&](int mm, int nn, G* w) { l.velocity(nn * (g.M + 1) + mm, w); };
-    // normal of the edge from vertex (ma, na) to vertex (mb, nb)
-    auto nrm = [&](int ma, int na, int mb, int nb, G* nr) {
-        G xa[2], xb[2];
-        vtx(ma, na, xa);
-        vtx(mb, nb, xb);
-        edge_normal(xa, xb, nr);
-    };
-    // ... and the mean velocity of its two vertices
+    // normal of the edge from vertex (ma, na) to vertex (mb, nb) and the mean velocity of its
+    // two vertices
     auto edge = [&](int ma, int na, int mb, int nb, G* nr, G* gv) {
         G xa[2], xb[2], va[2], vb[2];
         vtx(ma, na, xa);
@@ -477,6 +471,52 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const FsDir& 
         gv[0] = (va[0] + vb[0]) / 2.0;
         gv[1] = (va[1] + vb[1]) / 2.0;
     };
+    // the same from vertices and velocities already loaded
+    auto edge_of = [](const G* xa, const G* xb, const G* va, const G* vb, G* nr, G* gv) {
+        edge_normal(xa, xb, nr);
+        gv[0] = (va[0] + vb[0]) / 2.0;
+        gv[1] = (va[1] + vb[1]) / 2.0;
+    };
+
+    // 0. Plain FP64 geometry: the vertices and velocities of this cell's faces and area, and of
+    // the tile-edge face a thread below TM + TN evaluates, are loaded before the staging loop,
+    // so their latency overlaps the staging instead of following each face (clamped to the grid
+    // for threads without a cell or face).  Geometry with tangents (G = Dual) would hold twice
+    // the registers; it is loaded at each face, as vtx/vel calls there.
+    constexpr bool kEarlyCell = std::is_same<G, double>::value, kEarlyEdge = kEarlyCell;
+    G gx[4][2], gw[3][2];   // vertices BI, FI, FO, BO; velocities at FI, FO, BO
+    G ex[2][2], ew[2][2];   // tile-edge face: the edge from vertex ex[0] to vertex ex[1]
+    auto load_cell = [&](int mc, int nc) {
+        vtx(mc, nc, gx[BI]);
+        vtx(mc, nc + 1, gx[FI]);
+        vtx(mc + 1, nc + 1, gx[FO]);
+        vtx(mc + 1, nc, gx[BO]);
+        vel(mc, nc + 1, gw[0]);
+        vel(mc + 1, nc + 1, gw[1]);
+        vel(mc + 1, nc, gw[2]);
+    };
+    // the tile-edge face's vertices: threads below TM, the angular face n0-1 | n0 of row
+    // m0 + tid (edge bi -> bo); threads TM..TM+TN-1, the wall face of column n0 + tid - TM (edge
+    // bi -> fi) or its radial face m0-1 | m0 (edge fo -> bo of the cell below)
+    auto load_edge = [&](int mm, int nn) {
+        int ma, na, mb, nb;
+        if (tid < TM)
+            ma = mm, na = n0, mb = mm + 1, nb = n0;
+        else if (m0 == 0)
+            ma = 0, na = nn, mb = 0, nb = nn + 1;
+        else
+            ma = m0, na = nn + 1, mb = m0, nb = nn;
+        vtx(ma, na, ex[0]);
+        vtx(mb, nb, ex[1]);
+        vel(ma, na, ew[0]);
+        vel(mb, nb, ew[1]);
+    };
+    if constexpr (kEarlyCell) load_cell(m < g.M ? m : g.M - 1, n < g.N ? n : g.N - 1);
+    if constexpr (kEarlyEdge) {
+        if (tid < TM + TN)
+            load_edge(m0 + tid < g.M ? m0 + tid : g.M - 1,
+                      n0 + tid - TM < g.N ? n0 + tid - TM : g.N - 1);
+    }
 
     // 1. pressures, sound speeds and ghosts over the grown tile
     for (int i = tid; i < BOX; i += NT) {
@@ -533,8 +573,14 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const FsDir& 
     if (live) {
         const int bc = bidx(m, bn);
         G nr[2], nr2[2], gv[2];
-        edge(m, n + 1, m + 1, n + 1, nr, gv);     // fi -> fo
-        nrm(m + 1, n + 1, m, n + 1, nr2);         // R's bo -> bi
+        if constexpr (!kEarlyCell) {
+            vtx(m, n + 1, gx[FI]);
+            vtx(m + 1, n + 1, gx[FO]);
+            vel(m, n + 1, gw[0]);
+            vel(m + 1, n + 1, gw[1]);
+        }
+        edge_of(gx[FI], gx[FO], gw[0], gw[1], nr, gv);   // fi -> fo
+        edge_normal(gx[FO], gx[FI], nr2);                // R's bo -> bi
         const int bLL = bidx(m, bn - 1), bR = bidx(m, bn + 1), bRR = bidx(m, bn + 2);
         face_flux2_lds(P, stencil(bLL, bc, bR, bRR), sP[bLL], sP[bc], sP[bR], sP[bRR],
                        sC[bc], sC[bR], nr, nr2, gv, true, true,
@@ -555,10 +601,16 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const FsDir& 
     // ---- radial face m | m+1 (this cell's o face), or the far-field face
     if (live) {
         const int bc = bidx(m, bn);
+        if constexpr (!kEarlyCell) {
+            vtx(m + 1, n + 1, gx[FO]);
+            vtx(m + 1, n, gx[BO]);
+            vel(m + 1, n + 1, gw[1]);
+            vel(m + 1, n, gw[2]);
+        }
         if (m < g.M - 1) {
             G nr[2], nr2[2], gv[2];
-            edge(m + 1, n + 1, m + 1, n, nr, gv);     // fo -> bo
-            nrm(m + 1, n, m + 1, n + 1, nr2);         // R's bi -> fi
+            edge_of(gx[FO], gx[BO], gw[1], gw[2], nr, gv);   // fo -> bo
+            edge_normal(gx[BO], gx[FO], nr2);                // R's bi -> fi
             // m - 1 is the wall ghost row for m = 0, m + 2 the far-field row for m + 1 = M - 1
             const int bLL = bidx(m - 1, bn), bR = bidx(m + 1, bn), bRR = bidx(m + 2, bn);
             face_flux2_lds(P, stencil(bLL, bc, bR, bRR), sP[bLL], sP[bc], sP[bR], sP[bRR],
@@ -570,7 +622,7 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const FsDir& 
             box(m, bn, uc);
             box(g.M, bn, gh);
             G nr[2], gv[2];
-            edge(m + 1, n + 1, m + 1, n, nr, gv);
+            edge_of(gx[FO], gx[BO], gw[1], gw[2], nr, gv);
             face_flux(P, uc, uc, gh, gh, nr, gv, false, Fo);
 #pragma unroll
             for (int v = 0; v < 4; ++v) S[v] = S[v] + Fo[v];
@@ -587,11 +639,12 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const FsDir& 
     // They take the first wave(s) only: a branch taken by one or two lanes of every wave would
     // cost every wave a full face evaluation.  Their results reach the cells through LDS.
     if (tid < TM) {   // angular face n0-1 | n0; normal L -> R is minus cell (mm, n0)'s b normal
-        const int mm = m0 + tid, nn = n0;
+        const int mm = m0 + tid;
         if (mm < g.M) {
+            if constexpr (!kEarlyEdge) load_edge(mm, n0);
             G nr[2], nr2[2], gv[2];
-            edge(mm, nn, mm + 1, nn, nr, gv);           // bi -> bo
-            nrm(mm + 1, nn, mm, nn, nr2);               // the cell's bo -> bi
+            edge_of(ex[0], ex[1], ew[0], ew[1], nr, gv);   // bi -> bo
+            edge_normal(ex[1], ex[0], nr2);                // the cell's bo -> bi
             const int bLL = bidx(mm, 0), bL = bidx(mm, 1), bR = bidx(mm, 2), bRR = bidx(mm, 3);
             face_flux2_lds(P, stencil(bLL, bL, bR, bRR), sP[bLL], sP[bL], sP[bR], sP[bRR],
                            sC[bL], sC[bR], nr, nr2, gv, false, true, [](int, const T&) {},
@@ -600,19 +653,20 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const FsDir& 
     } else if (tid < TM + TN) {
         const int j = tid - TM, mm = m0, nn = n0 + j, bnn = j + 2;
         if (nn < g.N) {
+            if constexpr (!kEarlyEdge) load_edge(mm, nn);
             if (mm == 0) {   // wall face: central against the wall ghost
                 T uc[4], gh[4], Fi[4];
                 box(mm, bnn, uc);
                 box(-1, bnn, gh);
                 G nr[2], gv[2];
-                edge(0, nn, 0, nn + 1, nr, gv);          // bi -> fi
+                edge_of(ex[0], ex[1], ew[0], ew[1], nr, gv);   // bi -> fi
                 face_flux(P, uc, uc, gh, gh, nr, gv, false, Fi);
 #pragma unroll
                 for (int v = 0; v < 4; ++v) lds_put(sFi + v * W * TN, TN, j, Fi[v]);
             } else {         // radial face m0-1 | m0; normal L -> R is minus the cell's i normal
                 G nr[2], nr2[2], gv[2];
-                edge(mm, nn + 1, mm, nn, nr, gv);        // fo -> bo of L
-                nrm(mm, nn, mm, nn + 1, nr2);            // the cell's bi -> fi
+                edge_of(ex[0], ex[1], ew[0], ew[1], nr, gv);   // fo -> bo of L
+                edge_normal(ex[1], ex[0], nr2);                // the cell's bi -> fi
                 // mm - 2 is the wall ghost row for mm = 1, mm + 1 the far-field row for
                 // mm = M - 1
                 const int bLL = bidx(mm - 2, bnn), bL = bidx(mm - 1, bnn), bR = bidx(mm, bnn),
@@ -635,13 +689,9 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const FsDir& 
         if (tm == 0)
 #pragma unroll
             for (int v = 0; v < 4; ++v) S[v] = S[v] + lds_get<T>(sFi + v * W * TN, TN, tn);
-        G x[4][2];
-        vtx(m, n, x[BI]);
-        vtx(m, n + 1, x[FI]);
-        vtx(m + 1, n + 1, x[FO]);
-        vtx(m + 1, n, x[BO]);
-        const G d1x = x[FO][0] - x[BI][0], d1z = x[FO][1] - x[BI][1];
-        const G d2x = x[BO][0] - x[FI][0], d2z = x[BO][1] - x[FI][1];
+        if constexpr (!kEarlyCell) load_cell(m, n);
+        const G d1x = gx[FO][0] - gx[BI][0], d1z = gx[FO][1] - gx[BI][1];
+        const G d2x = gx[BO][0] - gx[FI][0], d2z = gx[BO][1] - gx[FI][1];
         const G ia = recip(sabs(d1x * d2z - d1z * d2x) / 2.0);
         T R[4];
 #pragma unroll
