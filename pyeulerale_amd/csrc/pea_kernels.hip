@@ -518,21 +518,25 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const FsDir& 
                       n0 + tid - TM < g.N ? n0 + tid - TM : g.N - 1);
     }
 
-    // 1. pressures, sound speeds and ghosts over the grown tile
-    for (int i = tid; i < BOX; i += NT) {
+    // 1. pressures, sound speeds and ghosts over the grown tile.  Box cell i holds the state of
+    // cell (cm, cn), or a ghost made from the state of the first or last row; its state is
+    // loaded from the row clamped to the grid.
+    auto stage_load = [&](int i, T* u) {
+        const int bm = i % BM, bn = i / BM;
+        const int cm = m0 - 2 + bm;
+        const int cn = wrap_any(n0 - 2 + bn, g.N);
+        l.state(cn * g.M + (cm < 0 ? 0 : (cm >= g.M ? g.M - 1 : cm)), u);
+    };
+    auto stage_cell = [&](int i, const T* u) {
         const int bm = i % BM, bn = i / BM;
         const int cm = m0 - 2 + bm;
         const int cn = wrap_any(n0 - 2 + bn, g.N);
         if (cm >= 0 && cm < g.M) {
-            T u[4];
-            l.state(cn * g.M + cm, u);
             cell_pc(u, sP[i], sC[i]);
 #pragma unroll
             for (int v = 0; v < 4; ++v) lds_put(sU + v * W * BOX, BOX, i, u[v]);
         } else if (cm == -1 || cm == g.M) {
-            const int mc = cm < 0 ? 0 : g.M - 1;
-            T u[4], gh[4];
-            l.state(cn * g.M + mc, u);
+            T gh[4];
             if (cm < 0) {
                 G nr[2], gv[2];
                 edge(0, cn, 0, cn + 1, nr, gv);       // wall face bi -> fi
@@ -546,6 +550,25 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const FsDir& 
 #pragma unroll
             for (int v = 0; v < 4; ++v) lds_put(sU + v * W * BOX, BOX, i, gh[v]);
             sP[i] = pressure_q(gh);
+        }
+    };
+    // FP64 states: every pass's loads are issued before the first is used (one round trip
+    // instead of one per pass; passes past the box load a valid cell and discard it).  States
+    // with tangents would hold a pass's 4 x W doubles more; they are loaded pass by pass.
+    if constexpr (std::is_same<T, double>::value) {
+        constexpr int NPASS = (BOX + NT - 1) / NT;
+        T u[NPASS][4];
+#pragma unroll
+        for (int k = 0; k < NPASS; ++k)
+            stage_load(tid + k * NT < BOX ? tid + k * NT : BOX - 1, u[k]);
+#pragma unroll
+        for (int k = 0; k < NPASS; ++k)
+            if (tid + k * NT < BOX) stage_cell(tid + k * NT, u[k]);
+    } else {
+        for (int i = tid; i < BOX; i += NT) {
+            T u[4];
+            stage_load(i, u);
+            stage_cell(i, u);
         }
     }
     __syncthreads();
