@@ -229,7 +229,8 @@ int pea_export_bsr(int M, int N, const double *JU, double shift_re, double shift
  * (core.py:594-598).  JU are the assembled blocks of pea_cs_linearize_odes. */
 const char *pea_resolvent_last_error(void);
 
-/* y_k = (J - s_k I) x_k for k < S, reading the blocks once for all vectors. */
+/* y_k = (J - s_k I) x_k for k < S, reading the blocks once for all vectors.  Grids of up to
+ * 2^24 - 1 cells (PEA_ERR_DIMS beyond). */
 int pea_block_multi(int M, int N, int S, const double *JU, const double *shifts,
                     int transposed, const double *x, double *y, void *stream);
 
