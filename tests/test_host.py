@@ -88,6 +88,11 @@ def test_library_rejects_bad_dimensions_without_gpu_work():
     assert rc == -2
     with pytest.raises(_lib.PeaError):
         _lib.call("pea_jvp_odes_multi", ctypes.byref(p), 0.5, 8, 8, 3, *([None] * 9))
+    # the block product's 32-bit lane offsets cover grids below 2^24 cells; the pointers are
+    # never dereferenced because the check comes first
+    dummy = ctypes.c_void_p(256)
+    rc = lib.pea_block_multi(4096, 4096, 1, dummy, dummy, 0, dummy, dummy, None)
+    assert rc == -1 and b"2^24" in lib.pea_resolvent_last_error()
 
 
 def test_vjp_workspace_size_is_positive():
