@@ -27,11 +27,12 @@ def sd():
     return s
 
 
-@pytest.mark.parametrize("S", [5, 11])
+@pytest.mark.parametrize("S", [5, 11, 32])
 @pytest.mark.parametrize("grid", ["33x17", "43x22"])
 def test_block_multi_matches_single_products(request, grid, S):
-    """S = 5 leaves a wave's second slot idle; S = 11 makes slots loop over a second vector;
-    43x22 has a cell count that is not a multiple of the kernel's 32-cell workgroups."""
+    """The kernel's four waves take vectors w, w + 4, ...: S = 5 leaves three waves one vector,
+    S = 11 gives some waves three and others two, S = 32 (the batched GMRES case) eight each.
+    43x22 has a cell count that is not a multiple of the kernel's 64-cell workgroups."""
     from pyeulerale_amd import _lib
     from pyeulerale_amd import kernels as K
     from pyeulerale_amd.gmres import _p
