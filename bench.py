@@ -69,6 +69,9 @@ def parse():
                          "a kernel trace of the Arnoldi steps")
     ap.add_argument("--only-multigrid", action="store_true",
                     help="only the 64-grid section (its JSON on one line), for a kernel trace")
+    ap.add_argument("--only-freq", action="store_true",
+                    help="only the 129x129 frequency-response section without the host SuperLU "
+                         "solve (its JSON on one line), for a kernel trace of the ring LU")
     return ap.parse_args()
 
 
@@ -202,10 +205,11 @@ def products(torch, K, A, Rv, thickness, mach, aoa, alpha, alpha_dot, far, reps)
     return res
 
 
-def freq_response(torch, superlu: bool, shifts: int = 32):
+def freq_response(torch, superlu: bool, shifts: int = 32, sequential: bool = True):
     """Steady state and 32-frequency pitch response on the reference's own 129x129 NACA0004 grid
     (examples/response/run.py, Ma 0.5, chord 10): all shifts in one batched ring-LU solve
-    against one shift at a time, plus the reference's host SuperLU solve of one shift."""
+    against one shift at a time (unless `sequential` is False), plus the reference's host
+    SuperLU solve of one shift."""
     from pyeulerale_amd.core import SpatialDiscretization
     from pyeulerale_amd.drivers import frequency_response, pseudo_transient_continuation
     grid = ROOT / "tests" / "golden" / "examples" / "naca0004_129x129.plot3d.gz"
@@ -223,6 +227,10 @@ def freq_response(torch, superlu: bool, shifts: int = 32):
     _, cl_b, cm_b = frequency_response(sd, k, chord=10.0, axis_location=0.0, log=None)
     sync()
     t_b = time.perf_counter() - t0
+    if not sequential:
+        return {"grid": "naca0004_129x129 (reference examples/grids), M=128, N=128",
+                "shifts": shifts, "ptc_iterations": len(hist), "ptc_s": t_ptc, "batched_s": t_b,
+                "cl_k0.01": [float(cl_b[0].real), float(cl_b[0].imag)]}
     t0 = time.perf_counter()
     _, cl_s, cm_s = frequency_response(sd, k, chord=10.0, axis_location=0.0, log=None,
                                        batched=False)
@@ -579,6 +587,9 @@ def main():
         return
     if args.only_multigrid:
         print(json.dumps(multigrid(torch, K)), flush=True)
+        return
+    if args.only_freq:
+        print(json.dumps(freq_response(torch, False, sequential=False)), flush=True)
         return
     mach, aoa = 0.6, 0.0
     omega, t_ = 0.2, 2.0
