@@ -1,13 +1,15 @@
 #!/usr/bin/env python
 """Benchmark of the spatial-discretization hot path on one MI355X.
 
-Workload (the metric's configuration): NACA 0004 O-grid, 257 angular x 129 radial vertices
-(M = 128 radial, N = 256 angular cells), chord 1, far field 20 chords, Ma 0.6, alpha 0 deg,
-grid rigidly pitching about the quarter chord (alpha_g = 0.5 deg sin(0.2 t) at t = 2), FP64.
-One step = one launch evaluating the residual R(Ma, U, X, V) of a batch of 64 seeded states
-U_inf (1 + 1e-3 N(0,1)) on that grid; ``value`` = residual evaluations per second.
+Workload (the metric's configuration): NACA 0004 O-grid, 1025 angular x 513 radial vertices
+(M = 512 radial, N = 1024 angular cells), chord 1, far field 20 chords, static grid, Ma 0.5,
+alpha 0 deg, FP64.  One step = one launch evaluating the residual R(Ma, U, X, V) of a batch of 8
+seeded states U_inf (1 + 1e-3 N(0,1)) on that grid (134 MB of states, 134 MB of residuals: more
+than the 256 MB Infinity Cache holds with the grid); ``value`` = residual evaluations per second.
 
 Also reported (same JSON line):
+
+* the 64-state batched residual on the 257x129 pitching grid, also beyond the Infinity Cache;
 
 * the roofline of the residual kernel, timed with HIP events on its stream, with its measured
   HBM traffic and FP64 rate from the committed counter summary;
@@ -44,6 +46,7 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 BATCH = 64
+HEAD_BATCH = 8
 
 
 def parse():
@@ -69,6 +72,9 @@ def parse():
                          "a kernel trace of the Arnoldi steps")
     ap.add_argument("--only-multigrid", action="store_true",
                     help="only the 64-grid section (its JSON on one line), for a kernel trace")
+    ap.add_argument("--only-head", action="store_true",
+                    help="only the headline residual launches (its JSON on one line), for a "
+                         "kernel trace of the timed kernel")
     ap.add_argument("--only-freq", action="store_true",
                     help="only the 129x129 frequency-response section without the host SuperLU "
                          "solve (its JSON on one line), for a kernel trace of the ring LU")
@@ -125,7 +131,7 @@ def pmc_figures() -> dict:
     try:
         for line in PMC_SUMMARY.read_text().splitlines():
             for key in ("traffic_bytes_per_launch", "fp64_flop_per_launch"):
-                if line.startswith(key + " k_face_fwd batched residual"):
+                if line.startswith(key + " k_face_fwd batched residual 1025x513"):
                     out[key] = float(line.rsplit(":", 1)[1])
     except OSError:
         pass
@@ -315,9 +321,14 @@ def profile_launches(torch, K):
     Rd = torch.empty_like(Ud)
     for _ in range(5):
         K.compute_odes_batch(p, mach, g, Xd, Vd, Ud, Rd)
-    X, V, U, M, N = make_case(1025, 513, 0.04, 0.5, 0.0, 0.0, 0.0, 30.0, 1)
+    X, V, U, M, N = make_case(1025, 513, 0.04, 0.5, 0.0, 0.0, 0.0, 20.0, HEAD_BATCH)
     g, p = K.Grid(M, N), K.Params(aoa_deg=0.0)
-    Xd, Vd, Ud = dev(X), dev(V), dev(U[0])
+    Xd, Vd, Ud = dev(X), dev(V), dev(U)
+    Rd = torch.empty_like(Ud)
+    for _ in range(5):
+        K.compute_odes_batch(p, 0.5, g, Xd, Vd, Ud, Rd)
+    del Rd
+    Ud = Ud[0].contiguous()
     rng = np.random.default_rng(1)
     cshape = (4, M, N)
     dU = dev(rng.standard_normal(cshape) + 1j * rng.standard_normal(cshape))
@@ -549,23 +560,52 @@ def gmres_steps(torch, K, S: int = 32, steps: int = 30):
             "ms_per_shift_step_batched_matrix_free": t_mf / (S * steps) * 1e3}
 
 
-def cpu_baseline(seconds, X, V, U, mach, aoa):
-    """The reference's compiled Fortran compute_odes on one host core (bounded sample)."""
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_worker(args):
+    """One host process: residual evaluations of the reference for `seconds` (bounded sample)."""
+    kind, seconds, X, V, U, mach, aoa = args
     from oracle.oracle import Oracle, Params
-    kind = "reference" if Oracle.available("ref") else "port"
     orc = Oracle("ref" if kind == "reference" else "port")
     prm = Params(aoa=aoa)
-    Xc, Vc = X.astype(complex), V.astype(complex)
-    Uc = [np.asfortranarray(U[i].astype(complex)) for i in range(U.shape[0])]
     n, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < seconds:
-        orc.compute_odes(prm, mach, Xc, Vc, Uc[n % len(Uc)])
+    while time.perf_counter() - t0 < seconds or n == 0:
+        orc.compute_odes(prm, mach, X, V, U)
         n += 1
-    dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "residual evals/s", "cores": 1, "kind": kind,
-            "sample": f"{n} single-state residual evaluations of the 257x129 workload "
-                      f"({'compiled reference Fortran, f90 compute_odes' if kind == 'reference' else 'C restatement'}) "
-                      f"on 1 host core in {dt:.1f} s"}
+    return n, time.perf_counter() - t0
+
+
+def cpu_baseline(seconds, X, V, U, mach, aoa, grid):
+    """The reference's compiled Fortran compute_odes (oracle/_ref) on the host: one core, like
+    the reference's serial build, and one process per core on every core this process may use
+    (at most 16), each evaluating the single-state residual for a bounded time.  Run before the
+    GPU is initialised, so the worker processes are plain forks of a CPU-only process."""
+    import multiprocessing as mp
+    from oracle.oracle import Oracle
+    kind = "reference" if Oracle.available("ref") else "port"
+    Xc, Vc, Uc = (np.asfortranarray(a.astype(complex)) for a in (X, V, U))
+    n1, t1 = _cpu_worker((kind, seconds, Xc, Vc, Uc, mach, aoa))
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    with mp.get_context("fork").Pool(cores) as pool:
+        res = pool.map(_cpu_worker, [(kind, seconds, Xc, Vc, Uc, mach, aoa)] * cores)
+    rate_all = sum(n / t for n, t in res)
+    src = ("compiled reference Fortran, f90 compute_odes" if kind == "reference"
+           else "C restatement")
+    return {"value": rate_all, "unit": "residual evals/s", "cores": cores, "kind": kind,
+            "cpu_model": _cpu_model(),
+            "sample": f"single-state residual evaluations of the {grid} workload ({src}): "
+                      f"{cores} processes, one per core, {sum(n for n, _ in res)} evaluations "
+                      f"in {seconds:.0f} s each",
+            "one_core": {"value": n1 / t1, "cores": 1,
+                         "sample": f"{n1} evaluations on 1 core in {t1:.1f} s"}}
 
 
 def main():
@@ -577,7 +617,6 @@ def main():
     import torch
 
     from pyeulerale_amd import kernels as K
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.profile:
         profile_launches(torch, K)
         progress("profile launches done")
@@ -591,11 +630,14 @@ def main():
     if args.only_freq:
         print(json.dumps(freq_response(torch, False, sequential=False)), flush=True)
         return
-    mach, aoa = 0.6, 0.0
-    omega, t_ = 0.2, 2.0
-    amp = np.deg2rad(0.5)
-    alpha, alpha_dot = amp * np.sin(omega * t_), amp * omega * np.cos(omega * t_)
-    X, V, U, M, N = make_case(257, 129, 0.04, mach, aoa, alpha, alpha_dot, 20.0, BATCH)
+    # headline workload: NACA0004 1025x513, static grid, Ma 0.5, alpha 0, 8 states per launch
+    mach, aoa = 0.5, 0.0
+    X, V, U, M, N = make_case(1025, 513, 0.04, mach, aoa, 0.0, 0.0, 20.0, HEAD_BATCH)
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds, X, V, U[0], mach, aoa,
+                                                "1025x513")
+    if cpu is not None:
+        progress("cpu baseline done")
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
     g, p = K.Grid(M, N), K.Params(aoa_deg=aoa)
     Xd = torch.as_tensor(np.ascontiguousarray(np.swapaxes(X, -1, -2))).cuda()
     Vd = torch.as_tensor(np.ascontiguousarray(np.swapaxes(V, -1, -2))).cuda()
@@ -603,9 +645,8 @@ def main():
     Rd = torch.empty_like(Ud)
     step = lambda: K.compute_odes_batch(p, mach, g, Xd, Vd, Ud, Rd)
     # The K timed launches are replayed from one HIP graph, and so are the warm-up launches after
-    # the first, which runs eagerly so that nothing is loaded during capture.  Launched one by
-    # one with events between them, the host adds ~10 us of gap to an ~80 us kernel.  The events
-    # bracket the timed replay on the stream the kernels run on.
+    # the first, which runs eagerly so that nothing is loaded during capture.  The events bracket
+    # the timed replay on the stream the kernels run on.
     K._lib.load()
     if args.warmup > 0:
         step()
@@ -630,11 +671,28 @@ def main():
     wall = time.perf_counter() - t0
     kern_ms = ev0.elapsed_time(ev1) / args.steps
     C, P = M * N, (M + 1) * (N + 1)
-    bytes_launch = BATCH * 64 * C + 32 * P
+    bytes_launch = HEAD_BATCH * 64 * C + 32 * P
     achieved = bytes_launch / (kern_ms * 1e-3) / 1e9
-    value = BATCH * args.steps / wall
+    value = HEAD_BATCH * args.steps / wall
+    del Ud, Rd, graphs
+    if args.only_head:
+        print(json.dumps({"value": value, "ms_per_step": wall / args.steps * 1e3,
+                          "kernel_ms": kern_ms, "steps": args.steps,
+                          "achieved_GBs": achieved}), flush=True)
+        return
 
-    progress(f"residual batch: {kern_ms:.4f} ms per launch")
+    # the 257x129 pitching workload of the products (Ma 0.6, rigid pitch about the quarter chord)
+    mach, aoa = 0.6, 0.0
+    omega, t_ = 0.2, 2.0
+    amp = np.deg2rad(0.5)
+    alpha, alpha_dot = amp * np.sin(omega * t_), amp * omega * np.cos(omega * t_)
+    X, V, U, M, N = make_case(257, 129, 0.04, mach, aoa, alpha, alpha_dot, 20.0, BATCH)
+    g, p = K.Grid(M, N), K.Params(aoa_deg=aoa)
+    Xd = torch.as_tensor(np.ascontiguousarray(np.swapaxes(X, -1, -2))).cuda()
+    Vd = torch.as_tensor(np.ascontiguousarray(np.swapaxes(V, -1, -2))).cuda()
+    Ud = torch.as_tensor(np.ascontiguousarray(np.swapaxes(U, -1, -2))).cuda()
+    Rd = torch.empty_like(Ud)
+    progress(f"residual 1025x513 x{HEAD_BATCH}: {kern_ms:.4f} ms per launch")
     cache = residual_beyond_cache(torch, K, p, mach, g, Xd, Vd, Ud, Rd, alpha, alpha_dot)
     progress("residual beyond the Infinity Cache done")
     extra = {"residual_beyond_infinity_cache": cache,
@@ -643,7 +701,7 @@ def main():
     progress("products 257x129 done")
     if not args.no_large:
         extra["products_1025x513"] = products(torch, K, 1025, 513, 0.04, 0.5, 0.0, 0.0, 0.0,
-                                              30.0, 10)
+                                              20.0, 10)
         progress("products 1025x513 done")
     extra["products_all_inputs_257x129"] = all_input_products(torch, K)
     progress("all-input products done")
@@ -665,16 +723,17 @@ def main():
         "value": value, "unit": "residual evals/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "NACA0004 O-grid 257x129 vertices (M=128, N=256), pitching, "
-                               "Ma 0.6, batch of 64 states per launch",
-                   "batch": BATCH, "grid": "257x129"},
+        "config": {"workload": "NACA0004 O-grid 1025x513 vertices (M=512, N=1024), far field "
+                               "20 chords, static, Ma 0.5, alpha 0, batch of 8 seeded states "
+                               "per launch (402 MB of states and residuals)",
+                   "batch": HEAD_BATCH, "grid": "1025x513"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": pmc.get("traffic_bytes_per_launch"),
                      "kernel": "k_face_fwd<double, double, 16, 16, LdReal, StoreRealB>",
                      "kernel_ms": kern_ms, "algorithmic_bytes_per_launch": bytes_launch,
-                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/"
-                                       "pmc_face_kernels_1025x513.md",
+                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE of the 8-state "
+                                       "1025x513 launch, profiles/pmc_face_kernels_1025x513.md",
                      "fp64_tflops": (pmc["fp64_flop_per_launch"] / (kern_ms * 1e-3) / 1e12
                                      if "fp64_flop_per_launch" in pmc else None),
                      "fp64_peak_tflops": FP64_PEAK_TFLOPS},
@@ -684,8 +743,8 @@ def main():
         "speedup_grid": "1025x513" if "products_1025x513" in extra else "257x129",
         "extra": extra,
     }
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds, X, V, U[:4], mach, aoa)
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
     print(json.dumps(line))
 
 

@@ -129,23 +129,43 @@ class Oracle:
         return self._call_state_fn("compute_odes_wrt_mach", prm, mach, X, V, U,
                                    _fzeros((4, 1, M, N), float))
 
-    def compute_odes_wrt_states(self, prm, mach, X, V, U):
-        X, V, U = _fcopy(X), _fcopy(V), _fcopy(U)
+    def _masked(self, name, prm, mach, X, V, U, mask, out, *extra):
+        """Port-only: the Jacobian blocks of the cells where ``mask`` (M, N) is true; the blocks
+        of other cells stay zero."""
+        if self.kind != "port":
+            raise NotImplementedError("masked Jacobians exist in the C restatement only")
         M, N = U.shape[1:]
-        return self._call_state_fn("compute_odes_wrt_states", prm, mach, X, V, U,
-                                   _fzeros((4, 4, M, N, 9), float))
+        mk = np.asfortranarray(np.asarray(mask, dtype=np.uint8).reshape(M, N))
+        f = self._fn(name)
+        f.argtypes = [_P, _c.c_double, _c.c_double, _c.c_int, _c.c_int, _P, _P, _P, _P] + \
+            [_c.c_int] * len(extra) + [_P]
+        f(self._cfg(prm), complex(mach).real, complex(mach).imag, M, N, _ptr(X), _ptr(V),
+          _ptr(U), _ptr(mk), *extra, _ptr(out))
+        return out
 
-    def compute_odes_wrt_vertices(self, prm, mach, X, V, U):
+    def compute_odes_wrt_states(self, prm, mach, X, V, U, mask=None):
         X, V, U = _fcopy(X), _fcopy(V), _fcopy(U)
         M, N = U.shape[1:]
-        return self._call_state_fn("compute_odes_wrt_vertices", prm, mach, X, V, U,
-                                   _fzeros((4, 2, M, N, 4), float))
+        out = _fzeros((4, 4, M, N, 9), float)
+        if mask is not None:
+            return self._masked("compute_odes_wrt_states_masked", prm, mach, X, V, U, mask, out)
+        return self._call_state_fn("compute_odes_wrt_states", prm, mach, X, V, U, out)
 
-    def compute_odes_wrt_velocities(self, prm, mach, X, V, U):
+    def compute_odes_wrt_vertices(self, prm, mach, X, V, U, mask=None):
         X, V, U = _fcopy(X), _fcopy(V), _fcopy(U)
         M, N = U.shape[1:]
-        return self._call_state_fn("compute_odes_wrt_velocities", prm, mach, X, V, U,
-                                   _fzeros((4, 2, M, N, 4), float))
+        out = _fzeros((4, 2, M, N, 4), float)
+        if mask is not None:
+            return self._masked("compute_odes_wrt_grid_masked", prm, mach, X, V, U, mask, out, 0)
+        return self._call_state_fn("compute_odes_wrt_vertices", prm, mach, X, V, U, out)
+
+    def compute_odes_wrt_velocities(self, prm, mach, X, V, U, mask=None):
+        X, V, U = _fcopy(X), _fcopy(V), _fcopy(U)
+        M, N = U.shape[1:]
+        out = _fzeros((4, 2, M, N, 4), float)
+        if mask is not None:
+            return self._masked("compute_odes_wrt_grid_masked", prm, mach, X, V, U, mask, out, 1)
+        return self._call_state_fn("compute_odes_wrt_velocities", prm, mach, X, V, U, out)
 
     def _apply(self, name, M, N, J, din, out_shape):
         J = _fcopy(J, float)
@@ -278,3 +298,59 @@ class Oracle:
         got = f(M, N, _ptr(J), _ptr(data), _ptr(idx), _ptr(ptr))
         assert got == nnzb
         return data, idx, ptr
+
+    # ----------------------------------------------------------------- second order
+    def inner_product_gradient(self, prm, mach, X, V, U, of: str, along: str, wrt: str,
+                               d_of, d_along, mask=None, step_scale: float = 1.0):
+        """The reference's second-order gradients grad_wrt <d_of, d(of)/d(along) . d_along>
+        (core.py:718-1275), by its own rule (core.py:749-758): central differences along
+        d_along of the complex-step Jacobian of ``of`` wrt ``wrt``, transposed onto conj(d_of),
+        four passes (sign +-1, real and imaginary part of d_along with factors 1 and 1j) and
+        step = 1e-6 sqrt(1 + |along|) / |d_along| (Frobenius norms).
+
+        of: "odes" (along states/vertices/velocities, wrt mach/states/vertices) or "forces"
+        (along and wrt states/vertices).  Arrays in the reference's layout.  ``mask`` (M, N)
+        restricts the complex-step Jacobians of the odes to those cells (port only); the result
+        is then exact only where every contributing row was evaluated.  ``step_scale`` != 1
+        scales the reference's step (a probe for cells where the differences straddle a switch
+        of the biased maximum, f90:111-120; the reference always uses 1)."""
+        X, V, U = _fcopy(X), _fcopy(V), _fcopy(U)
+        M, N = U.shape[1:]
+        base = {"states": U, "vertices": X, "velocities": V}[along]
+        d_along = np.asarray(d_along, dtype=complex)
+        w = np.conj(np.asarray(d_of, dtype=complex))
+        step = step_scale * 1e-6 * (1.0 + np.linalg.norm(base)) ** 0.5 / np.linalg.norm(d_along)
+        total = None
+        for sign in (1.0, -1.0):
+            for part, factor in ((np.real, 1.0), (np.imag, 1.0j)):
+                pert = base + sign * step * part(d_along)
+                Xp, Vp, Up = (pert if along == "vertices" else X,
+                              pert if along == "velocities" else V,
+                              pert if along == "states" else U)
+                if of == "odes":
+                    if wrt == "mach":
+                        J = self.compute_odes_wrt_mach(prm, mach, Xp, Vp, Up)
+                        g = self.apply_odes_wrt_mach_rev(J, w)
+                    elif wrt == "states":
+                        J = self.compute_odes_wrt_states(prm, mach, Xp, Vp, Up, mask=mask)
+                        g = self.apply_odes_wrt_states_rev(J, w)
+                    elif wrt == "vertices":
+                        J = self.compute_odes_wrt_vertices(prm, mach, Xp, Vp, Up, mask=mask)
+                        g = self.apply_odes_wrt_vertices_rev(J, w)
+                    else:
+                        raise ValueError(wrt)
+                elif of == "forces":
+                    if along == "velocities":
+                        raise ValueError("forces do not depend on velocities")
+                    if wrt == "states":
+                        g = self.apply_forces_wrt_states_rev(
+                            self.compute_forces_wrt_states(Xp, Up), w, M)
+                    elif wrt == "vertices":
+                        g = self.apply_forces_wrt_vertices_rev(
+                            self.compute_forces_wrt_vertices(Xp, Up), w, M)
+                    else:
+                        raise ValueError(wrt)
+                else:
+                    raise ValueError(of)
+                total = sign * factor * g if total is None else total + sign * factor * g
+        return total / 2.0 / step

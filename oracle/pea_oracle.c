@@ -17,7 +17,8 @@
  *   dF/dX                J[d + 2*e + 4*n + 4*N*s]             s<2
  * Indices are 0-based here (the reference is 1-based).
  *
- * Parity status per function (how each is pinned) is listed in oracle/README.md.
+ * Each function is pinned by tests/test_oracle.py: fixtures written by the compiled reference
+ * (oracle/_ref), fresh random cases against it, invariants and finite differences.
  */
 #include <complex.h>
 #include <math.h>
@@ -240,12 +241,12 @@ void orc_compute_odes_wrt_mach(const orc_params *prm, double mach_re, double mac
 
 /* f90:666-750: 9 real 4x4 blocks per cell by complex step.  Blocks whose neighbour does not
  * exist (radial boundary) are left untouched, as in the reference. */
-void orc_compute_odes_wrt_states(const orc_params *prm, double mach_re, double mach_im, int M, int N,
-                                 const cplx *X, const cplx *V, const cplx *U, double *J) {
-    const cplx mach = CMPLX(mach_re, mach_im);
+static void odes_wrt_states(const orc_params *prm, cplx mach, int M, int N, const cplx *X,
+                            const cplx *V, const cplx *U, const unsigned char *mask, double *J) {
     const long C = (long)M * N;
     for (int n = 0; n < N; ++n)
         for (int m = 0; m < M; ++m) {
+            if (mask && !mask[m + (long)M * n]) continue;
             stencil s;
             gather(M, N, X, V, U, m, n, &s);
             int flag = boundary_flag(M, m + 1);
@@ -271,12 +272,27 @@ void orc_compute_odes_wrt_states(const orc_params *prm, double mach_re, double m
         }
 }
 
+void orc_compute_odes_wrt_states(const orc_params *prm, double mach_re, double mach_im, int M, int N,
+                                 const cplx *X, const cplx *V, const cplx *U, double *J) {
+    odes_wrt_states(prm, CMPLX(mach_re, mach_im), M, N, X, V, U, NULL, J);
+}
+
+/* The same blocks for the cells with mask[m + M*n] != 0 only (other cells untouched): lets the
+ * checks evaluate the complex-step Jacobian on a sample of a large grid. */
+void orc_compute_odes_wrt_states_masked(const orc_params *prm, double mach_re, double mach_im,
+                                        int M, int N, const cplx *X, const cplx *V,
+                                        const cplx *U, const unsigned char *mask, double *J) {
+    odes_wrt_states(prm, CMPLX(mach_re, mach_im), M, N, X, V, U, mask, J);
+}
+
 /* f90:933-993 (which = 0) and f90:1087-1143 (which = 1): 4 real 4x2 blocks per cell */
 static void odes_wrt_grid(const orc_params *prm, cplx mach, int M, int N, const cplx *X,
-                          const cplx *V, const cplx *U, double *J, int which) {
+                          const cplx *V, const cplx *U, const unsigned char *mask, double *J,
+                          int which) {
     const long C = (long)M * N;
     for (int n = 0; n < N; ++n)
         for (int m = 0; m < M; ++m) {
+            if (mask && !mask[m + (long)M * n]) continue;
             stencil s;
             gather(M, N, X, V, U, m, n, &s);
             int flag = boundary_flag(M, m + 1);
@@ -301,13 +317,20 @@ static void odes_wrt_grid(const orc_params *prm, cplx mach, int M, int N, const 
 void orc_compute_odes_wrt_vertices(const orc_params *prm, double mach_re, double mach_im, int M, int N,
                                    const cplx *X, const cplx *V, const cplx *U, double *J) {
     const cplx mach = CMPLX(mach_re, mach_im);
-    odes_wrt_grid(prm, mach, M, N, X, V, U, J, 0);
+    odes_wrt_grid(prm, mach, M, N, X, V, U, NULL, J, 0);
+}
+
+/* Vertex (which = 0) or velocity (which = 1) blocks of the masked cells only. */
+void orc_compute_odes_wrt_grid_masked(const orc_params *prm, double mach_re, double mach_im,
+                                      int M, int N, const cplx *X, const cplx *V, const cplx *U,
+                                      const unsigned char *mask, int which, double *J) {
+    odes_wrt_grid(prm, CMPLX(mach_re, mach_im), M, N, X, V, U, mask, J, which);
 }
 
 void orc_compute_odes_wrt_velocities(const orc_params *prm, double mach_re, double mach_im, int M, int N,
                                      const cplx *X, const cplx *V, const cplx *U, double *J) {
     const cplx mach = CMPLX(mach_re, mach_im);
-    odes_wrt_grid(prm, mach, M, N, X, V, U, J, 1);
+    odes_wrt_grid(prm, mach, M, N, X, V, U, NULL, J, 1);
 }
 
 /* f90:604-621 */

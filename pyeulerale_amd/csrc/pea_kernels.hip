@@ -42,10 +42,16 @@ int hip_fail(hipError_t e, const char* what) {
         if (e_ != hipSuccess) return hip_fail(e_, what);        \
     } while (0)
 
+// The buffer loaders address one array with 32-bit byte offsets and a 32-bit record count
+// (buf_rsrc below): a complex cell field spans 64 C bytes, so C < 2^25 cells keeps every offset
+// and the count of the largest array below 2^31.
+constexpr long long PEA_MAX_CELLS = (1LL << 25) - 1;
+
 int check_dims(int M, int N) {
     if (M < 4 || N < 5) return fail(PEA_ERR_DIMS, "need M >= 4 and N >= 5 (got M=%d N=%d)", M, N);
-    if ((long long)(M + 1) * (N + 1) > (1LL << 30))
-        return fail(PEA_ERR_DIMS, "grid too large (M=%d N=%d)", M, N);
+    if ((long long)M * N > PEA_MAX_CELLS || (long long)(M + 1) * (N + 1) > 2 * PEA_MAX_CELLS)
+        return fail(PEA_ERR_DIMS, "grid too large for 32-bit buffer offsets: M*N = %lld cells, "
+                    "at most %lld (M=%d N=%d)", (long long)M * N, PEA_MAX_CELLS, M, N);
     return PEA_OK;
 }
 
@@ -108,7 +114,7 @@ PEA_HD int vtx_index(const Grid& g, int m, int n, int vtx) {
 // Loads through a buffer descriptor: the element offset is one 32-bit lane value shared by
 // every plane of an array, and the plane is a scalar offset, so no 64-bit address is formed per
 // load.  The descriptor covers `bytes` (offsets beyond read 0); word 3 is the gfx9 raw-buffer
-// format.  Arrays up to 4 GiB.
+// format.  Arrays below 2 GiB: check_dims bounds the cell count accordingly.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const double* p, long bytes) {
     return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, 0x00020000);
 }

@@ -30,7 +30,8 @@ def pseudo_transient_continuation(sd, rtol: float = 1e-9, max_iter: int = 100,
     """Drives ``sd.states`` to a steady state; returns (iteration, ||R||_inf, relative).
 
     The shift falls with the residual, so the last steps are Newton steps (shift -> 0).  The
-    default ``solver="direct"`` is the reference's sparse LU and converges quadratically there.
+    default ``solver="direct"`` (an exact GPU factorisation, as the reference's sparse LU) converges
+    quadratically there.
     ``solver="gmres"`` is exact only while GMRES converges, which for the nearly imaginary
     spectrum of the JST Jacobian needs large shifts (|s| of order 10 on a 65x33 grid); use it
     for the early, strongly shifted steps or with a loose ``rtol`` as an inexact Newton method.

@@ -51,6 +51,22 @@ def bytes_per_shift(M: int, N: int) -> int:
     return 3 * M * (4 * N) ** 2 * 16
 
 
+def check_fits(M: int, N: int, shifts: int, free_bytes: int) -> None:
+    """Raises :class:`PeaError` before any allocation when the factorisation of ``shifts``
+    shifts needs more than ``free_bytes`` of device memory (a ring LU of 1025x513 vertices needs
+    412 GB for one shift)."""
+    need = bytes_per_shift(M, N) * max(1, int(shifts))
+    if need > free_bytes:
+        raise _lib.PeaError(
+            f"ring LU of {M}x{N} cells needs {need / 1e9:.1f} GB for {shifts} shift(s), "
+            f"{free_bytes / 1e9:.1f} GB of device memory are free")
+
+
+def _free_bytes() -> int:
+    free, _ = torch.cuda.mem_get_info()
+    return int(free + torch.cuda.memory_reserved() - torch.cuda.memory_allocated())
+
+
 class RingLU:
     """Factorisation of (dR/dU - s_k I), or of the adjoint system, for the shifts s_k."""
 
@@ -58,6 +74,7 @@ class RingLU:
                  refine: int = 1, dense: str = "auto"):
         if JU.device.type != "cuda":
             raise RuntimeError("RingLU runs on the GPU (pea_ring_band / pea_ring_couple)")
+        check_fits(g.M, g.N, len(np.atleast_1d(shifts)), _free_bytes())
         K._need(JU, (9, 4, 4, g.N, g.M), torch.float64, "JU")
         self.g = g
         self.JU = JU
@@ -200,11 +217,14 @@ class RingLU:
 
 def solve_shifts(g: K.Grid, JU: torch.Tensor, b: torch.Tensor, shifts, *, adjoint: bool = False,
                  max_bytes: float = 64 * 2 ** 30, refine: int = 1):
-    """Solves all shifts, in chunks whose factorisations fit in ``max_bytes`` of device memory.
-    Returns (x, relative residual per shift)."""
+    """Solves all shifts, in chunks whose factorisations fit in ``max_bytes`` of device memory
+    (and in the free device memory).  Raises :class:`PeaError` before factorising anything when
+    not even one shift fits.  Returns (x, relative residual per shift)."""
     shifts = np.asarray(shifts, dtype=complex).reshape(-1)
     per = bytes_per_shift(g.M, g.N)
-    chunk = max(1, int(max_bytes // per))
+    free = 0.9 * _free_bytes()
+    check_fits(g.M, g.N, 1, int(free))
+    chunk = max(1, int(min(float(max_bytes), free) // per))
     xs, res = [], []
     for i in range(0, len(shifts), chunk):
         lu = RingLU(g, JU, shifts[i:i + chunk], adjoint=adjoint, refine=refine)

@@ -9,8 +9,11 @@ fixtures store inputs and the reference's outputs:
   tiny.npz  -- NACA 0012, 9 x 8 vertices (M=7, N=8), rigidly pitching, Ma 0.5, alpha 1.25 deg:
                R, F, every complex-step Jacobian and fwd/rev products along seeded directions.
   c1.npz    -- NACA 0012, 129 x 65 vertices (M=64, N=128), static, Ma 0.5, alpha 1.25 deg,
-               U = U_inf (1 + 1e-3 N(0,1)) from seed 0: R, F and four dR/dU products along
-               seeded real directions (complex step, the reference's block method).
+               U = U_inf (1 + 1e-3 N(0,1)) from seed 0: R, F and the dR/dU products along the
+               16 seeded real directions of the configuration (complex step, the reference's
+               block method).  The directions are default_rng(16).standard_normal((16,4,M,N));
+               they are regenerated by the tests, which check them against the stored
+               checksum (dirs_head, dirs_sum) first.
   zeros.npz -- NACA 0004, 17 x 9 vertices, static, Ma 0.5, alpha 0: rho*w == 0 exactly, so
                |n.j/rho| is evaluated at signed zeros on the z = 0 grid line (derivative
                convention of abs_c, f90:97-102).
@@ -32,6 +35,11 @@ sys.path.insert(0, str(ROOT / "tests"))
 
 from helpers import case  # noqa: E402
 from oracle.oracle import Oracle, Params  # noqa: E402
+
+
+def c1_directions(M, N):
+    """The 16 seeded real directions of the C1 configuration."""
+    return np.random.default_rng(16).standard_normal((16, 4, M, N))
 
 
 def full(orc, prm, mach, X, V, U, rng):
@@ -75,11 +83,12 @@ def main():
     X, V, U = case(129, 65, thickness=0.12, mach=0.5, aoa=1.25, seed=0)
     M, N = U.shape[1:]
     JU = ref.compute_odes_wrt_states(prm, 0.5, X, V, U)
-    dirs = np.random.default_rng(16).standard_normal((4, 4, M, N))
+    dirs = c1_directions(M, N)
     prods = np.stack([ref.apply_odes_wrt_states_fwd(JU, d).real for d in dirs])
     np.savez_compressed(HERE / "c1.npz", X=X, V=V, U=U, mach=0.5, aoa=1.25,
                         R=ref.compute_odes(prm, 0.5, X, V, U).real,
-                        F=ref.compute_forces(prm, X, U).real, dirs=dirs, prods=prods)
+                        F=ref.compute_forces(prm, X, U).real, dirs_head=dirs[:, 0, 0, :8],
+                        dirs_sum=dirs.sum(axis=(1, 2, 3)), prods=prods)
 
     prm0 = Params(aoa=0.0)
     X, V, U = case(17, 9, thickness=0.04, mach=0.5, aoa=0.0, seed=3)

@@ -56,3 +56,59 @@ dev_verts = dev_cells
 def host(t) -> np.ndarray:
     """Canonical storage -> reference layout numpy array."""
     return np.swapaxes(t.detach().cpu().numpy(), -1, -2)
+
+
+def c1_directions(d) -> np.ndarray:
+    """The 16 seeded real directions of the C1 fixture (tests/golden/make_golden.py),
+    regenerated and checked against the checksum stored with it."""
+    U = np.asarray(d["U"])
+    M, N = U.shape[1:]
+    dirs = np.random.default_rng(16).standard_normal((16, 4, M, N))
+    assert np.array_equal(dirs[:, 0, 0, :8], d["dirs_head"]), "direction generator changed"
+    assert np.allclose(dirs.sum(axis=(1, 2, 3)), d["dirs_sum"], rtol=0, atol=1e-9)
+    return dirs
+
+
+# ---------------------------------------------------------------- sampled checks on large grids
+def sample_mask(M: int, N: int, width: int = 8) -> np.ndarray:
+    """Cells at which the oracle evaluates complex-step Jacobians on a large grid: radial bands
+    of ``width`` rings at the wall, at mid radius and at the far field, and angular bands of
+    ``width`` columns across the periodic seam and at the leading edge (n = N/2)."""
+    mask = np.zeros((M, N), bool)
+    h = width // 2
+    mask[:width] = True
+    mask[M // 2 - h:M // 2 + h] = True
+    mask[M - width:] = True
+    mask[:, np.r_[-h:h] % N] = True
+    mask[:, N // 2 - h:N // 2 + h] = True
+    return mask
+
+
+def inner_cells(mask: np.ndarray) -> np.ndarray:
+    """Cells whose transposed products are complete from the sampled rows: every row r whose
+    stencil (the cross of reach 2, periodic in n) contains the cell is in ``mask``."""
+    ok = mask.copy()
+    for dm in (-2, -1, 1, 2):
+        sh = np.ones_like(mask)                 # rows beyond the radial ends do not exist
+        if dm > 0:
+            sh[:-dm] = mask[dm:]
+        else:
+            sh[-dm:] = mask[:dm]
+        ok &= sh
+    for dn in (-2, -1, 1, 2):
+        ok &= np.roll(mask, -dn, axis=1)
+    return ok
+
+
+def inner_vertices(mask: np.ndarray) -> np.ndarray:
+    """Vertices (M+1, N+1) all of whose cells are in ``mask``: vertex (m, n) is corner bi of
+    cell (m, n), fi of (m, n-1), fo of (m-1, n-1) and bo of (m-1, n), without wrap-around (the
+    seam column n = N belongs to cell N-1 only)."""
+    M, N = mask.shape
+    ok = np.ones((M + 1, N + 1), bool)
+    pad = np.ones((M + 2, N + 2), bool)         # cells outside the grid do not exist
+    pad[1:-1, 1:-1] = mask
+    for dm in (0, 1):
+        for dn in (0, 1):
+            ok &= pad[dm:dm + M + 1, dn:dn + N + 1]
+    return ok

@@ -1,0 +1,362 @@
+"""GPU parity at the benchmark configurations' real sizes, through the C ABI.
+
+* C2  NACA0004 257x129, pitching, Ma 0.6: the hand-written tangents and adjoints along every
+      input (U, X, V, Ma) and the force products (U, X), each for 64 seeded directions, against
+      the oracle's complex-step blocks (<= 1e-10) and against central differences of the GPU
+      residual and forces (h = 1e-7, <= 1e-5 as the reference's own tests).
+* C3  NACA0004 513x257, far field 30c, Ma 0.5, alpha 1 deg: grad_{Ma,U,X} <dR, dR/dU dU> and
+      grad_{U,X} <dF, dF/dX dX> through the drop-in class against the reference's own rule
+      (central differences of complex-step Jacobians, core.py:749-758) run by the oracle on a
+      sample of the grid.
+* C4  the 64 uneven O-grids of the multi-grid launch: residual, tangent and adjoint bitwise
+      equal to one single-grid call per grid, and residuals of three grids against the oracle.
+* C5  NACA0004 1025x513, Ma 0.5: R and F against the oracle (<= 1e-12) and the products along
+      U, X, V and Ma against the oracle's complex-step blocks on a sample of the grid.
+
+Sampled checks: on the larger grids the oracle evaluates complex-step Jacobian rows only on
+bands of rings (wall, mid radius, far field) and of columns (seam, leading edge); forward
+products are compared on those rows, transposed products on the cells and vertices whose every
+contributing row lies in the sample.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from helpers import crandn, host, inner_cells, inner_vertices, rel, sample_mask
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    return torch.as_tensor(np.ascontiguousarray(np.swapaxes(np.asarray(a), -1, -2))).cuda()
+
+
+def say(msg):
+    print(f"[configs] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def orc():
+    from oracle.oracle import Oracle
+    return Oracle("port")
+
+
+# ------------------------------------------------------------------------------------------ C2
+@pytest.fixture(scope="module")
+def c2(orc):
+    from oracle.oracle import Params
+    from pyeulerale_amd import workloads as W
+    c = W.config("C2")
+    X, V, U = c.X, c.V, c.U[0]
+    prm = Params(aoa=c.aoa)
+    say("C2 oracle blocks")
+    blocks = dict(JU=orc.compute_odes_wrt_states(prm, c.mach, X, V, U),
+                  JX=orc.compute_odes_wrt_vertices(prm, c.mach, X, V, U),
+                  JV=orc.compute_odes_wrt_velocities(prm, c.mach, X, V, U),
+                  JMa=orc.compute_odes_wrt_mach(prm, c.mach, X, V, U),
+                  JFU=orc.compute_forces_wrt_states(X, U),
+                  JFX=orc.compute_forces_wrt_vertices(X, U))
+    return c, blocks
+
+
+def test_c2_products_along_every_input_64_directions(orc, c2):
+    from pyeulerale_amd import kernels as K
+    c, B = c2
+    M, N, S = c.M, c.N, 64
+    g, p = K.Grid(M, N), K.Params(aoa_deg=c.aoa)
+    X, V, U = dev(c.X), dev(c.V), dev(c.U[0])
+    rng = np.random.default_rng(64)
+    dU, w = crandn(rng, (S, 4, M, N)), crandn(rng, (S, 4, M, N))
+    dX, dV = crandn(rng, (S, 2, M + 1, N + 1)), crandn(rng, (S, 2, M + 1, N + 1))
+    dMa, wF = crandn(rng, (S,)), crandn(rng, (S, 2, N))
+    say("C2 tangents and adjoints, 64 directions each")
+    jU = host(K.jvp_odes_multi(p, c.mach, g, X, V, U, dU=dev(dU)))
+    gU = host(K.vjp_odes_states_multi(p, c.mach, g, X, V, U, dev(w)))
+    worst = {}
+
+    def check(key, got, ref, bar=1e-10):
+        e = rel(got, ref)
+        worst[key] = max(worst.get(key, 0.0), e)
+        assert e <= bar, (key, e)
+
+    for k in range(S):
+        check("jvp_U", jU[k], orc.apply_odes_wrt_states_fwd(B["JU"], dU[k]))
+        check("vjp_U", gU[k], orc.apply_odes_wrt_states_rev(B["JU"], w[k]))
+        check("jvp_X", host(K.jvp_odes(p, c.mach, g, X, V, U, dX=dev(dX[k]))),
+              orc.apply_odes_wrt_vertices_fwd(B["JX"], dX[k]))
+        check("jvp_V", host(K.jvp_odes(p, c.mach, g, X, V, U, dV=dev(dV[k]))),
+              orc.apply_odes_wrt_vertices_fwd(B["JV"], dV[k]))
+        check("jvp_Ma", host(K.jvp_odes(p, c.mach, g, X, V, U, dmach=complex(dMa[k]))),
+              orc.apply_odes_wrt_mach_fwd(B["JMa"], dMa[k]))
+        out = K.vjp_odes(p, c.mach, g, X, V, U, dev(w[k]),
+                         want=("states", "vertices", "velocities", "mach"))
+        # one seed per launch is bitwise the batched launch; the launch that also produces the
+        # grid and Mach outputs uses the cell-based adjoint, equal to round-off
+        check("vjp_U_bitwise", host(K.vjp_odes(p, c.mach, g, X, V, U, dev(w[k]))["states"]),
+              gU[k], 0.0)
+        check("vjp_U_all_outputs", host(out["states"]), orc.apply_odes_wrt_states_rev(B["JU"], w[k]))
+        check("vjp_X", host(out["vertices"]), orc.apply_odes_wrt_vertices_rev(B["JX"], w[k]))
+        check("vjp_V", host(out["velocities"]), orc.apply_odes_wrt_vertices_rev(B["JV"], w[k]))
+        check("vjp_Ma", out["mach"].cpu().numpy(), orc.apply_odes_wrt_mach_rev(B["JMa"], w[k]))
+        check("fjvp_U", K.jvp_forces(g, X, U, dU=dev(dU[k])).cpu().numpy(),
+              orc.apply_forces_wrt_states_fwd(B["JFU"], dU[k]))
+        check("fjvp_X", K.jvp_forces(g, X, U, dX=dev(dX[k])).cpu().numpy(),
+              orc.apply_forces_wrt_vertices_fwd(B["JFX"], dX[k]))
+        fv = K.vjp_forces(g, X, U, torch.as_tensor(wF[k]).cuda())
+        check("fvjp_U", host(fv["states"]), orc.apply_forces_wrt_states_rev(B["JFU"], wF[k], M))
+        check("fvjp_X", host(fv["vertices"]),
+              orc.apply_forces_wrt_vertices_rev(B["JFX"], wF[k], M))
+    say("C2 worst relative errors " + ", ".join(f"{k} {v:.1e}" for k, v in worst.items()))
+
+
+def test_c2_products_against_central_differences(c2):
+    """The method of tests/test_linearization.py:78-330: central differences of R and F with
+    h = 1e-7 (1e-8 along the vertices, as the reference's own vertex test: the trailing-edge
+    cells of the cosine-clustered 257-point surface are ~1e-4 chord wide) along real and
+    imaginary parts of the directions, to the reference's rtol of 1e-5 (norm-wise here).  The differences are not exact at the kinks of the scheme: at ~1e-3 state
+    noise a few of the 130k faces have a pressure-sensor difference |p_r - 2p + p_l|
+    (f90:198-217) within 1e-7 of zero, where the step straddles the kink of |.|; the measured
+    norm-wise differences are a few 1e-6."""
+    from pyeulerale_amd import kernels as K
+    c, _ = c2
+    M, N = c.M, c.N
+    g, p = K.Grid(M, N), K.Params(aoa_deg=c.aoa)
+    X, V, U = dev(c.X), dev(c.V), dev(c.U[0])
+    rng = np.random.default_rng(65)
+    h = 1e-7
+    R = lambda m=c.mach, Xa=X, Va=V, Ua=U: K.compute_odes(p, m, g, Xa, Va, Ua)
+    F = lambda Xa=X, Ua=U: K.compute_forces(g, Xa, Ua)
+
+    def cfd(fn, base, d, key, h=h):
+        parts = []
+        for part in (d.real, d.imag):
+            pt = torch.as_tensor(np.ascontiguousarray(np.swapaxes(part, -1, -2))).cuda()
+            parts.append((fn(**{key: base + h * pt}) - fn(**{key: base - h * pt})) / (2 * h))
+        return parts[0] + 1j * parts[1]
+
+    say("C2 central differences")
+    for k in range(4):
+        dU, dX, dV = crandn(rng, (4, M, N)), crandn(rng, (2, M + 1, N + 1)), \
+            crandn(rng, (2, M + 1, N + 1))
+        dMa = complex(rng.standard_normal())
+        fd_m = (R(m=c.mach + h) - R(m=c.mach - h)) / (2 * h) * dMa
+        errs = {
+            "U": rel(host(K.jvp_odes(p, c.mach, g, X, V, U, dU=dev(dU))),
+                     host(cfd(R, U, dU, "Ua"))),
+            "X": rel(host(K.jvp_odes(p, c.mach, g, X, V, U, dX=dev(dX))),
+                     host(cfd(R, X, dX, "Xa", 1e-8))),
+            "V": rel(host(K.jvp_odes(p, c.mach, g, X, V, U, dV=dev(dV))),
+                     host(cfd(R, V, dV, "Va"))),
+            "Ma": rel(host(K.jvp_odes(p, c.mach, g, X, V, U, dmach=dMa)), host(fd_m)),
+            "F_U": rel(K.jvp_forces(g, X, U, dU=dev(dU)).cpu().numpy(),
+                       cfd(F, U, dU, "Ua").cpu().numpy()),
+            "F_X": rel(K.jvp_forces(g, X, U, dX=dev(dX)).cpu().numpy(),
+                       cfd(F, X, dX, "Xa", 1e-8).cpu().numpy()),
+        }
+        say(f"C2 direction {k}: " + ", ".join(f"{a} {b:.1e}" for a, b in errs.items()))
+        for a, b in errs.items():
+            assert b <= 1e-5, (a, b)
+
+
+# ------------------------------------------------------------------------------------------ C3
+def _c3_methods(sd, dR, dU, dF, dX):
+    return {
+        ("odes", "states", "mach"):
+            sd.compute_odes_wrt_states_inner_product_wrt_mach(dR, dU).cpu().numpy(),
+        ("odes", "states", "states"):
+            sd.compute_odes_wrt_states_inner_product_wrt_states(dR, dU).cpu().numpy(),
+        ("odes", "states", "vertices"):
+            sd.compute_odes_wrt_states_inner_product_wrt_vertices(dR, dU).cpu().numpy(),
+        ("forces", "vertices", "states"):
+            sd.compute_forces_wrt_vertices_inner_product_wrt_states(dF, dX).cpu().numpy(),
+        ("forces", "vertices", "vertices"):
+            sd.compute_forces_wrt_vertices_inner_product_wrt_vertices(dF, dX).cpu().numpy(),
+    }
+
+
+@pytest.mark.parametrize("noise", [1e-2, 1e-3])
+def test_c3_second_order_against_reference_rule(orc, noise):
+    """Exact tangent-over-adjoint against the reference's central differences of complex-step
+    Jacobians, on the C3 grid.  The rule's step moves every state entry by ~3e-8.  With the
+    configuration's 1e-3 state noise that straddles a switch of the biased maximum
+    (f90:111-120) at a few cells, where the reference's result is then wrong by orders of
+    magnitude; those cells are found by running the rule again with a third of its step and
+    are left out (at most 0.1% of the checked cells).  With 1e-2 noise no switch lies that close
+    and every sampled cell is compared."""
+    from oracle.oracle import Params
+    from pyeulerale_amd import workloads as W
+    from pyeulerale_amd.core import SpatialDiscretization
+    c = W.config("C3", noise=noise)
+    X, V, U = c.X, c.V, c.U[0]
+    M, N = c.M, c.N
+    rng = np.random.default_rng(3)
+    dR, dU, dF = crandn(rng, (4, M, N)), crandn(rng, (4, M, N)), crandn(rng, (2, N))
+    dX = crandn(rng, (2, M + 1, N + 1))
+    dX[:, M, :] = 0.0                                   # the far-field ring stays put
+    sd = SpatialDiscretization(vertices=X, angle_of_attack=c.aoa, mach_number=c.mach)
+    sd.states[:] = torch.as_tensor(U).cuda()
+    say(f"C3 noise {noise}: GPU second-order gradients")
+    got = _c3_methods(sd, dR, dU, dF, dX)
+    mask = sample_mask(M, N)
+    cells, verts = inner_cells(mask), inner_vertices(mask)
+    prm = Params(aoa=c.aoa)
+    for (of, along, wrt), g in got.items():
+        d_of, d_al = (dR, dU) if of == "odes" else (dF, dX)
+        say(f"C3 noise {noise}: oracle rule {of}/{along}/{wrt}")
+        ref = orc.inner_product_gradient(prm, c.mach, X, V, U, of, along, wrt, d_of, d_al,
+                                         mask=mask if of == "odes" else None)
+        if of == "forces" and wrt == "vertices":
+            # forces are linear in the vertices: the exact second derivative is zero and the
+            # rule returns its round-off
+            scale = np.linalg.norm(d_of) * np.linalg.norm(d_al)
+            assert np.linalg.norm(g) <= 1e-12 * scale
+            assert np.linalg.norm(ref) <= 1e-6 * scale
+            continue
+        sel = (cells if wrt == "states" else verts if wrt == "vertices" else None) \
+            if of == "odes" else None
+        keep = np.ones(np.shape(ref)[1:], bool) if sel is None else sel.copy()
+        if noise < 1e-2 and of == "odes":
+            probe = orc.inner_product_gradient(prm, c.mach, X, V, U, of, along, wrt, d_of, d_al,
+                                               mask=mask, step_scale=1 / 3)
+            if wrt == "mach":
+                if rel(ref, probe) > 1e-6:
+                    say("C3: the rule's Mach gradient straddles a switch; not compared")
+                    continue
+            else:
+                per = np.abs(ref - probe).sum(axis=0) / (np.abs(probe).sum(axis=0) + 1e-300)
+                bad = keep & (per > 1e-6)
+                assert bad.sum() <= max(1, keep.sum() // 1000), bad.sum()
+                say(f"C3 noise {noise}: {of}/{along}/{wrt}: {int(bad.sum())} of "
+                    f"{int(keep.sum())} cells straddle a switch, left out")
+                keep &= ~bad
+        if wrt == "mach":
+            e = rel(g, ref)
+        else:
+            e = rel(g[:, keep], ref[:, keep])
+        say(f"C3 noise {noise}: {of}/{along}/{wrt} rel {e:.2e}")
+        assert e <= 1e-6, (of, along, wrt, e)
+
+
+# ------------------------------------------------------------------------------------------ C4
+def test_c4_grid_family_bitwise_and_oracle(orc):
+    from oracle.oracle import Params
+    from pyeulerale_amd import kernels as K
+    from pyeulerale_amd import workloads as W
+    fam = W.grid_family(64, seed=0)
+    items = [(K.Params(aoa_deg=m.case.aoa), m.case.mach, dev(m.case.X), dev(m.case.V),
+              dev(m.case.U[0])) for m in fam]
+    say(f"C4: {len(items)} grids, {sum(m.case.M * m.case.N for m in fam)} cells")
+    gen = torch.Generator(device="cuda").manual_seed(4)
+    dUs = [torch.randn(it[4].shape, dtype=torch.complex128, device="cuda", generator=gen)
+           for it in items]
+    ws = [torch.randn(it[4].shape, dtype=torch.complex128, device="cuda", generator=gen)
+          for it in items]
+    Rs = K.compute_odes_multi(items)
+    tang = K.jvp_odes_multigrid([it + (d,) for it, d in zip(items, dUs)])
+    adj = K.vjp_odes_multigrid([it + (w,) for it, w in zip(items, ws)])
+    for i, (p, mach, X, V, U) in enumerate(items):
+        g = K.Grid(U.shape[2], U.shape[1])
+        R1 = K.compute_odes(p, mach, g, X, V, U)
+        assert torch.equal(Rs[i], R1), i
+        R2 = torch.empty_like(R1)
+        dR = K.jvp_odes(p, mach, g, X, V, U, dU=dUs[i], R=R2)
+        assert torch.equal(tang[i][0], R2) and torch.equal(tang[i][1], dR), i
+        gU = K.vjp_odes(p, mach, g, X, V, U, ws[i])["states"]
+        assert torch.equal(adj[i], gU), i
+    sizes = [m.case.M * m.case.N for m in fam]
+    for i in (int(np.argmin(sizes)), int(np.argsort(sizes)[len(sizes) // 2]),
+              int(np.argmax(sizes))):
+        m = fam[i]
+        say(f"C4: grid {i} ({m.angular}x{m.radial}) against the oracle")
+        ref = orc.compute_odes(Params(aoa=m.case.aoa), m.case.mach, m.case.X, m.case.V,
+                               m.case.U[0]).real
+        assert rel(host(Rs[i]), ref) <= 1e-12
+
+
+# ------------------------------------------------------------------------------------------ C5
+@pytest.fixture(scope="module")
+def c5(orc):
+    from oracle.oracle import Params
+    from pyeulerale_amd import workloads as W
+    c = W.config("C5")
+    X, V, U = c.X, c.V, c.U[0]
+    prm = Params(aoa=c.aoa)
+    mask = sample_mask(c.M, c.N)
+    say("C5 oracle: residual, forces, sampled blocks")
+    ref = dict(R=orc.compute_odes(prm, c.mach, X, V, U).real,
+               F=orc.compute_forces(prm, X, U).real,
+               JU=orc.compute_odes_wrt_states(prm, c.mach, X, V, U, mask=mask),
+               JX=orc.compute_odes_wrt_vertices(prm, c.mach, X, V, U, mask=mask),
+               JV=orc.compute_odes_wrt_velocities(prm, c.mach, X, V, U, mask=mask),
+               JMa=orc.compute_odes_wrt_mach(prm, c.mach, X, V, U),
+               JFU=orc.compute_forces_wrt_states(X, U),
+               JFX=orc.compute_forces_wrt_vertices(X, U))
+    return c, mask, ref
+
+
+def test_c5_residual_and_forces(c5):
+    from pyeulerale_amd import kernels as K
+    c, _, ref = c5
+    g, p = K.Grid(c.M, c.N), K.Params(aoa_deg=c.aoa)
+    X, V, U = dev(c.X), dev(c.V), dev(c.U[0])
+    e_r = rel(host(K.compute_odes(p, c.mach, g, X, V, U)), ref["R"])
+    e_f = rel(K.compute_forces(g, X, U).cpu().numpy(), ref["F"])
+    # the batched launch of the benchmark gives the same residual for every state
+    Ub = dev(np.stack([c.U[0], c.U[0]]))
+    Rb = host(K.compute_odes_batch(p, c.mach, g, X, V, Ub))
+    say(f"C5: residual rel {e_r:.2e}, forces rel {e_f:.2e}")
+    assert e_r <= 1e-12 and e_f <= 1e-12
+    assert rel(Rb[1], ref["R"]) <= 1e-12
+
+
+def test_c5_products_on_sampled_rows(orc, c5):
+    from pyeulerale_amd import kernels as K
+    c, mask, B = c5
+    M, N = c.M, c.N
+    g, p = K.Grid(M, N), K.Params(aoa_deg=c.aoa)
+    X, V, U = dev(c.X), dev(c.V), dev(c.U[0])
+    rng = np.random.default_rng(5)
+    dU, dX, dV, w = crandn(rng, (4, M, N)), crandn(rng, (2, M + 1, N + 1)), \
+        crandn(rng, (2, M + 1, N + 1)), crandn(rng, (4, M, N))
+    dMa, wF = complex(rng.standard_normal()), crandn(rng, (2, N))
+    cells, verts = inner_cells(mask), inner_vertices(mask)
+    say("C5: products")
+    errs = {
+        "jvp_U": rel(host(K.jvp_odes(p, c.mach, g, X, V, U, dU=dev(dU)))[:, mask],
+                     orc.apply_odes_wrt_states_fwd(B["JU"], dU)[:, mask]),
+        "jvp_X": rel(host(K.jvp_odes(p, c.mach, g, X, V, U, dX=dev(dX)))[:, mask],
+                     orc.apply_odes_wrt_vertices_fwd(B["JX"], dX)[:, mask]),
+        "jvp_V": rel(host(K.jvp_odes(p, c.mach, g, X, V, U, dV=dev(dV)))[:, mask],
+                     orc.apply_odes_wrt_vertices_fwd(B["JV"], dV)[:, mask]),
+        "jvp_Ma": rel(host(K.jvp_odes(p, c.mach, g, X, V, U, dmach=dMa)),
+                      orc.apply_odes_wrt_mach_fwd(B["JMa"], dMa)),
+    }
+    out = K.vjp_odes(p, c.mach, g, X, V, U, dev(w), want=("states", "vertices", "velocities",
+                                                        "mach"))
+    errs["vjp_U"] = rel(host(out["states"])[:, cells],
+                        orc.apply_odes_wrt_states_rev(B["JU"], w)[:, cells])
+    errs["vjp_X"] = rel(host(out["vertices"])[:, verts],
+                        orc.apply_odes_wrt_vertices_rev(B["JX"], w)[:, verts])
+    errs["vjp_V"] = rel(host(out["velocities"])[:, verts],
+                        orc.apply_odes_wrt_vertices_rev(B["JV"], w)[:, verts])
+    errs["vjp_Ma"] = rel(out["mach"].cpu().numpy(), orc.apply_odes_wrt_mach_rev(B["JMa"], w))
+    errs["fjvp_U"] = rel(K.jvp_forces(g, X, U, dU=dev(dU)).cpu().numpy(),
+                         orc.apply_forces_wrt_states_fwd(B["JFU"], dU))
+    errs["fjvp_X"] = rel(K.jvp_forces(g, X, U, dX=dev(dX)).cpu().numpy(),
+                         orc.apply_forces_wrt_vertices_fwd(B["JFX"], dX))
+    fv = K.vjp_forces(g, X, U, torch.as_tensor(wF).cuda())
+    errs["fvjp_U"] = rel(host(fv["states"]), orc.apply_forces_wrt_states_rev(B["JFU"], wF, M))
+    errs["fvjp_X"] = rel(host(fv["vertices"]),
+                         orc.apply_forces_wrt_vertices_rev(B["JFX"], wF, M))
+    # the batched forms of the states products give the same result
+    jb = host(K.jvp_odes_multi(p, c.mach, g, X, V, U, dU=dev(np.stack([dU, w]))))
+    vb = host(K.vjp_odes_states_multi(p, c.mach, g, X, V, U, dev(np.stack([w, dU]))))
+    errs["jvp_U_batched"] = rel(jb[0][:, mask], orc.apply_odes_wrt_states_fwd(B["JU"], dU)[:, mask])
+    errs["vjp_U_batched"] = rel(vb[0][:, cells],
+                                orc.apply_odes_wrt_states_rev(B["JU"], w)[:, cells])
+    say("C5 relative errors " + ", ".join(f"{k} {v:.1e}" for k, v in errs.items()))
+    for k, e in errs.items():
+        assert e <= 1e-10, (k, e)

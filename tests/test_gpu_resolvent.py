@@ -247,7 +247,17 @@ def test_ring_lu_matches_superlu(sd, adjoint):
     b = crandn(rng, (len(shifts), 4, sd.num_radial, sd.num_angular))
     print(f"[resolvent] ring LU adjoint={adjoint}", flush=True)
     x = sd.solve_odes_wrt_states_batched(b, shifts, adjoint=adjoint).cpu().numpy()
-    assert float(sd.last_residual.max()) <= 1e-12, sd.last_residual
+    # Backward stability: the residual of a partial-pivoting LU with one step of refinement is
+    # bounded by a small multiple of eps |A| |x| (norm-wise; |A| <= |J|_F + |s| sqrt(n) in
+    # Frobenius norm).  The multiple 16 leaves room for the order of the batched arithmetic.
+    JF = float(torch.linalg.vector_norm(sd._get_blocks()["U"]))
+    n = 4 * sd.num_radial * sd.num_angular
+    eps = np.finfo(float).eps
+    for k, s in enumerate(shifts):
+        bound = 16 * eps * (JF + abs(s) * np.sqrt(n)) * np.linalg.norm(x[k]) / np.linalg.norm(b[k])
+        print(f"[resolvent] shift {s}: residual {float(sd.last_residual[k]):.2e}, "
+              f"backward-error bound {bound:.2e}", flush=True)
+        assert float(sd.last_residual[k]) <= bound, (k, float(sd.last_residual[k]), bound)
     solve = sd.solve_odes_wrt_states_adj if adjoint else sd.solve_odes_wrt_states_fwd
     for k, s in enumerate(shifts):
         xd = solve(b[k], shift=s, solver="superlu").cpu().numpy()

@@ -3,6 +3,7 @@ C ABI library (loads and exports every symbol include/pea.h declares) and argume
 
 from __future__ import annotations
 
+import ctypes
 import re
 from pathlib import Path
 
@@ -93,6 +94,20 @@ def test_library_rejects_bad_dimensions_without_gpu_work():
     dummy = ctypes.c_void_p(256)
     rc = lib.pea_block_multi(4096, 4096, 1, dummy, dummy, 0, dummy, dummy, None)
     assert rc == -1 and b"2^24" in lib.pea_resolvent_last_error()
+    # the face kernels' buffer loads use 32-bit byte offsets: 2^25 cells and more are refused
+    # before any launch (a complex cell field of 2^25 cells spans 2 GiB)
+    d = [dummy] * 6
+    for M, N in ((8192, 4096), (4096, 8193)):
+        rc = lib.pea_compute_odes(ctypes.byref(p), 0.5, M, N, *d[:4], None)
+        assert rc == -1 and b"32-bit buffer offsets" in lib.pea_last_error()
+        rc = lib.pea_jvp_odes(ctypes.byref(p), 0.5, M, N, *d[:3], None, None, None, dummy,
+                              dummy, None, None)
+        assert rc == -1
+        rc = lib.pea_vjp_odes(ctypes.byref(p), 0.5, M, N, *d[:4], dummy, None, None, None,
+                              None, None)
+        assert rc == -1
+    assert lib.pea_compute_odes(ctypes.byref(p), 0.5, 8191, 4096, None, None, None, None,
+                                None) == -2            # just below the limit: dims accepted
 
 
 def test_vjp_workspace_size_is_positive():
@@ -144,3 +159,15 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode == 2
     line = json.loads(r.stdout.strip().splitlines()[-1])
     assert line["n_gpus"] == 2 and "one GPU" in line["error"]
+
+
+def test_ring_lu_refuses_what_does_not_fit():
+    from pyeulerale_amd import _lib
+    from pyeulerale_amd.ringlu import bytes_per_shift, check_fits
+    assert bytes_per_shift(512, 1024) > 400e9          # 1025x513 vertices: 412 GB per shift
+    with pytest.raises(_lib.PeaError, match="412"):
+        check_fits(512, 1024, 1, 288 * 10 ** 9)
+    check_fits(128, 128, 32, 288 * 10 ** 9)            # 129x129, 32 shifts: 52 GB
+    with pytest.raises(_lib.PeaError):
+        check_fits(128, 128, 32, 40 * 10 ** 9)
+
