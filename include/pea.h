@@ -143,6 +143,17 @@ int pea_vjp_odes(const pea_params *prm, double mach, int M, int N, const double 
                  const double *V, const double *U, const double *w, double *gU, double *gX,
                  double *gV, double *gmach, void *workspace, void *stream);
 
+/* pea_vjp_odes without a workspace argument and without any device allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  The Mach partial sums go to a static 2 MiB device array per
+ * GPU (131072 workgroup partials: grids whose pea_vjp_workspace_bytes exceeds 2 MiB are refused
+ * with PEA_ERR_DIMS when gmach != NULL).  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (on different streams) race; calls in stream order do not.  Results are
+ * bitwise those of pea_vjp_odes. */
+int pea_vjp_odes_noalloc(const pea_params *prm, double mach, int M, int N, const double *X,
+                         const double *V, const double *U, const double *w, double *gU,
+                         double *gX, double *gV, double *gmach, void *stream);
+
 /* k seeds at once, states output only: w and gU are k consecutive complex arrays.  One launch of
  * the face adjoint (one seed per grid z) where N >= 18, else one adjoint per seed; each gU
  * bitwise equal to pea_vjp_odes of that seed.  The reference transposes one product per call

@@ -47,6 +47,7 @@ _SIGS = {
     "pea_jvp_forces": [_I, _I, _P, _P, _P, _P, _P, _P],
     "pea_vjp_odes": [_P, _D, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "pea_vjp_workspace_bytes": [_I, _I],
+    "pea_vjp_odes_noalloc": [_P, _D, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "pea_vjp_forces": [_I, _I, _P, _P, _P, _P, _P, _P],
     "pea_hvp_forces": [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P],
     "pea_hvp_odes": [_P, _D, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],

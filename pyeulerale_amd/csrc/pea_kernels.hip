@@ -2213,6 +2213,56 @@ int pea_vjp_odes(const pea_params* prm, double mach, int M, int N, const double*
                               gmach, workspace, as_stream(stream));
 }
 
+}  // extern "C"
+
+// Partial sums of the Mach adjoint for pea_vjp_odes_noalloc: one complex value per workgroup of
+// the cell-based adjoint.  Each GPU holds its own copy for the life of the process; host code
+// reaches it only through the address from hipGetSymbolAddress (g_scratch_ptr).
+constexpr long long kScratchBytes = 2LL << 20;   // 2 MiB: 131072 partial sums
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up once per device (with that device
+// current, as every call is made).
+int scratch_address(void** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return hip_fail(hipGetLastError(), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) return fail(PEA_ERR_ARG, "device %d beyond %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return PEA_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int pea_vjp_odes_noalloc(const pea_params* prm, double mach, int M, int N, const double* X,
+                         const double* V, const double* U, const double* w, double* gU,
+                         double* gX, double* gV, double* gmach, void* stream) {
+    if (int rc = check_dims(M, N)) return rc;
+    if (!prm || !X || !V || !U || !w) return fail(PEA_ERR_NULL, "null argument");
+    void* ws = nullptr;
+    if (gmach) {
+        const long long need = pea_vjp_workspace_bytes(M, N);
+        if (need > kScratchBytes)
+            return fail(PEA_ERR_DIMS, "pea_vjp_odes_noalloc: M=%d N=%d needs %lld bytes of "
+                        "partial sums, the static scratch holds %lld", M, N, need, kScratchBytes);
+        if (int rc = scratch_address(&ws)) return rc;
+    }
+    return launch_vjp<double>(*prm, LdReal{X, V, U, mach, make_grid(M, N)}, w, gU, gX, gV,
+                              gmach, ws, as_stream(stream));
+}
+
 int pea_vjp_odes_states_multi(const pea_params* prm, double mach, int M, int N, int k,
                               const double* X, const double* V, const double* U, const double* w,
                               double* gU, void* stream) {

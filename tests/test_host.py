@@ -171,3 +171,18 @@ def test_ring_lu_refuses_what_does_not_fit():
     with pytest.raises(_lib.PeaError):
         check_fits(128, 128, 32, 40 * 10 ** 9)
 
+
+def test_noalloc_adjoint_refuses_grids_beyond_its_scratch():
+    """pea_vjp_odes_noalloc keeps the Mach partial sums in a static 2 MiB array: a grid whose
+    partials need more is refused before anything is launched."""
+    from pyeulerale_amd import _lib
+    lib = _lib.load()
+    p = _lib.PeaParams(1.25, 1.0, 1 / 32, 2.0)
+    d = ctypes.c_void_p(256)
+    assert lib.pea_vjp_workspace_bytes(2048, 1024) > 2 * 2 ** 20
+    rc = lib.pea_vjp_odes_noalloc(ctypes.byref(p), 0.5, 2048, 1024, d, d, d, d, None, None,
+                                  None, d, None)
+    assert rc == -1 and b"static scratch" in lib.pea_last_error()
+    rc = lib.pea_vjp_odes_noalloc(ctypes.byref(p), 0.5, 8, 8, None, d, d, d, None, None, None,
+                                  None, None)
+    assert rc == -2
