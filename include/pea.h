@@ -282,6 +282,45 @@ int pea_dense_lu_factor(int S, int n, double *A, int *ipiv, int *info, void *str
 int pea_dense_lu_solve(int S, int n, int nrhs, const double *A, const int *ipiv, double *B,
                        void *stream);
 
+/* Nested-dissection direct resolvent: a multifrontal block LU of dR/dU - s I (or of the
+ * adjoint system (dR/dU)^T - conj(s) I), replacing the SuperLU factorisation and solve of
+ * core.py:562-564 and 596-598 on grids where the ring LU does not fit (its factors need
+ * 3 M (4N)^2 complex values per shift).  The elimination tree and every index map come from the
+ * host (pyeulerale_amd/nested.py); these calls do one tree level, all its fronts in one batch.
+ * A front is a dense L x L complex matrix, column-major, L = 4 (Pc + Bc): its separator cells
+ * (padded to Pc) then its ring cells (padded to Bc), 4 unknowns per cell; cells[k (Pc+Bc) + q]
+ * is the q-th cell of front k (-1 = padding).  All arrays are device arrays; int32 maps.
+ * pea_nd_assemble: F = the children's update matrices gathered at their ring positions (child
+ *   2k + c of front k indexes U_child, each (4 Bc_child)^2 column-major; child_pos
+ *   ((2k + c)(Pc+Bc) + q) = position of front cell q in that child's ring or -1), plus identity
+ *   on S's padding.  Every entry is written.
+ * pea_nd_blocks: adds the original 4x4 blocks of JU (the assembled blocks of
+ *   pea_cs_linearize_odes) listed in `entries` (ne x 5 int32: node + node0, front row cell,
+ *   front column cell, row cell, block 0..8), minus the shift on block 0's diagonal; adjoint != 0
+ *   adds the transposed block at the transposed position (the caller passes conj(s)).
+ * pea_nd_factor: LU with partial pivoting of F11 (P x P, P = 4 Pc), F12 <- F11^-1 F12 and
+ *   F22 <- F22 - F21 F12 (B = 4 Bc), in place for n fronts (rocSOLVER getrf/getrs, rocBLAS
+ *   ZGEMM, strided batched); ipiv n x P, info n.
+ * pea_nd_gather / pea_nd_scatter: out[k][r][4 q + v] <-> y[r ldy + v M N + cells[k(Pc+Bc) + off
+ *   + q]] for q < cnt (gather writes 0 for padding; scatter skips it).
+ * pea_nd_accumulate: y[r ldy + target[t]] -= sum_{e = ptr[t]}^{ptr[t+1]-1} W[(src_k[e] nrhs + r)
+ *   B + src_j[e]] for every target t and right-hand side r, the sum in the listed order. */
+const char *pea_nd_last_error(void);
+int pea_nd_assemble(int n, int Pc, int Bc, const int *cells, const int *child,
+                    const int *child_pos, int Bc_child, const double *U_child, double *F,
+                    void *stream);
+int pea_nd_blocks(int ne, const int *entries, int node0, int M, int N, const double *JU,
+                  double shift_re, double shift_im, int adjoint, int Pc, int Bc, double *F,
+                  void *stream);
+int pea_nd_factor(int n, int P, int B, double *F, int *ipiv, int *info, void *stream);
+int pea_nd_gather(int n, int cnt, int off, int Lc, const int *cells, int nrhs, int M, int N,
+                  const double *y, long long ldy, double *out, void *stream);
+int pea_nd_scatter(int n, int cnt, int off, int Lc, const int *cells, int nrhs, int M, int N,
+                   const double *T, double *y, long long ldy, void *stream);
+int pea_nd_accumulate(int ntarget, int nrhs, const int *ptr, const int *src_k, const int *src_j,
+                      const int *target, int B, const double *W, double *y, long long ldy,
+                      void *stream);
+
 /* GMRES: Givens update of column j of S Hessenberg matrices on the device (H (S, R+1, R), cs/sn
  * (S, R), g (S, R+1) complex row-major; est[s] = |g[s][j+1]|), so that Arnoldi steps need no
  * host round trip and can be captured in a HIP graph. */

@@ -83,10 +83,18 @@ _SIGS = {
     "pea_gs_orthogonalize": [_I, _I, _I, _P, _c.c_longlong, _c.c_longlong, _P, _c.c_longlong,
                              _I, _P, _P, _P, _P, _c.c_longlong, _P, _P],
     "pea_dense_lu_solve": [_I, _I, _I, _P, _P, _P, _P],
+    "pea_nd_last_error": [],
+    "pea_nd_assemble": [_I, _I, _I, _P, _P, _P, _I, _P, _P, _P],
+    "pea_nd_blocks": [_I, _P, _I, _I, _I, _P, _D, _D, _I, _I, _I, _P, _P],
+    "pea_nd_factor": [_I, _I, _I, _P, _P, _P, _P],
+    "pea_nd_gather": [_I, _I, _I, _I, _P, _I, _I, _I, _P, _c.c_longlong, _P, _P],
+    "pea_nd_scatter": [_I, _I, _I, _I, _P, _I, _I, _I, _P, _P, _c.c_longlong, _P],
+    "pea_nd_accumulate": [_I, _I, _P, _P, _P, _P, _I, _P, _P, _c.c_longlong, _P],
 }
 _RESTYPES = {"pea_last_error": _c.c_char_p, "pea_vjp_workspace_bytes": _c.c_longlong,
              "pea_multi_workspace_bytes": _c.c_longlong,
-             "pea_resolvent_last_error": _c.c_char_p, "pea_line_factor_bytes": _c.c_longlong}
+             "pea_resolvent_last_error": _c.c_char_p, "pea_line_factor_bytes": _c.c_longlong,
+             "pea_nd_last_error": _c.c_char_p}
 
 _RESOLVENT = {"pea_block_multi", "pea_line_factor", "pea_line_solve", "pea_line_solve_lanes", "pea_jacobi_factor",
               "pea_jacobi_solve", "pea_ring_band", "pea_ring_couple", "pea_dense_lu_factor",
@@ -120,7 +128,8 @@ def call(name: str, *args) -> int:
     """Calls ``name`` and raises :class:`PeaError` on a negative status."""
     rc = getattr(load(), name)(*args)
     if rc < 0:
-        src = load().pea_resolvent_last_error if name in _RESOLVENT else load().pea_last_error
+        src = (load().pea_resolvent_last_error if name in _RESOLVENT else
+               load().pea_nd_last_error if name.startswith("pea_nd_") else load().pea_last_error)
         msg = src().decode(errors="replace")
         raise PeaError(f"{name} failed ({rc}): {msg}")
     return rc

@@ -1,0 +1,140 @@
+"""GPU tests of the nested-dissection direct resolvent (pyeulerale_amd/nested.py), through the C
+ABI (pea_nd_*):
+
+* on random blocks with the stencil's sparsity, against a dense solve (forward and adjoint);
+* on the complex-step Jacobian of a 33x17 grid, against the reference's SuperLU solve
+  (core.py:533-599) at shifts from 0 to 3 + 0.5i, forward and adjoint;
+* on the 257x129 grid, against the ring block LU;
+* at 1025x513 (where the ring LU would need 412 GB), by the residual of the solve with the
+  exact block operator, relative to the backward-error scale eps |A| |x| / |b|.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from helpers import case, crandn, rel
+from test_nested_symbolic import dense, random_blocks
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[nested] {msg}", flush=True)
+
+
+def storage(a):
+    return torch.as_tensor(np.ascontiguousarray(np.swapaxes(np.asarray(a), -1, -2))).cuda()
+
+
+@pytest.mark.parametrize("M,N,leaf", [(8, 16, 16), (12, 26, 24), (16, 32, 64)])
+@pytest.mark.parametrize("adjoint", [False, True])
+def test_random_blocks_against_dense_solve(M, N, leaf, adjoint):
+    from pyeulerale_amd import kernels as K
+    from pyeulerale_amd.nested import Factorization
+    say(f"random blocks M={M} N={N} leaf={leaf} adjoint={adjoint}")
+    JU = random_blocks(M, N, seed=M * N)
+    shift = complex(0.3, 1.7)
+    A = dense(JU, M, N) - shift * np.eye(4 * M * N)
+    if adjoint:
+        A = A.conj().T
+    rng = np.random.default_rng(2)
+    b = crandn(rng, (3, 4 * M * N))
+    ref = np.linalg.solve(A, b.T).T
+    f = Factorization(K.Grid(M, N), torch.as_tensor(JU).cuda(), shift, adjoint=adjoint,
+                      leaf=leaf)
+    assert f.singular_pivots == 0
+    x = f.solve(torch.as_tensor(b.reshape(3, 4, N, M)).cuda()).cpu().numpy().reshape(3, -1)
+    assert rel(x, ref) <= 1e-10, rel(x, ref)
+
+
+@pytest.fixture(scope="module")
+def sd33():
+    from pyeulerale_amd.core import SpatialDiscretization
+    X, V, U = case(33, 17, alpha=0.02, alpha_dot=0.01, seed=4)
+    sd = SpatialDiscretization(vertices=X, angle_of_attack=1.25, mach_number=0.5)
+    sd.velocities[:] = torch.as_tensor(V).cuda()
+    sd.states[:] = torch.as_tensor(U).cuda()
+    sd.linearize()
+    return sd
+
+
+@pytest.mark.parametrize("adjoint", [False, True])
+def test_jacobian_against_superlu(sd33, adjoint):
+    from pyeulerale_amd.nested import Factorization
+    rng = np.random.default_rng(3)
+    shifts = [0.0, 0.05j, 1j, 3 + 0.5j]
+    g = sd33._grid
+    JU = sd33._get_blocks()["U"]
+    for s in shifts:
+        say(f"33x17 shift {s} adjoint={adjoint}")
+        b = crandn(rng, (4, sd33.num_radial, sd33.num_angular))
+        f = Factorization(g, JU, s, adjoint=adjoint, leaf=16)
+        x = f.solve(storage(b)[None])[0].transpose(-1, -2).cpu().numpy()
+        solve = sd33.solve_odes_wrt_states_adj if adjoint else sd33.solve_odes_wrt_states_fwd
+        ref = solve(b, shift=s, solver="superlu").cpu().numpy()
+        assert rel(x, ref) <= 1e-10, (s, rel(x, ref))
+
+
+def test_257x129_against_ring_lu():
+    from pyeulerale_amd import kernels as K
+    from pyeulerale_amd import workloads as W
+    from pyeulerale_amd.nested import Factorization
+    from pyeulerale_amd.ringlu import RingLU
+    c = W.config("C2")
+    g, p = K.Grid(c.M, c.N), K.Params(aoa_deg=c.aoa)
+    X, V, U = storage(c.X), storage(c.V), storage(c.U[0])
+    JU = K.cs_linearize_odes(p, c.mach, g, X, V, U, want=("states",))[0]
+    rng = np.random.default_rng(5)
+    b = torch.as_tensor(crandn(rng, (2, 4, c.N, c.M))).cuda()
+    for s in (0.05j, 2j):
+        say(f"257x129 shift {s}")
+        f = Factorization(g, JU, s)
+        x = f.solve(b)
+        xr = RingLU(g, JU, [s, s]).solve(b)
+        assert rel(x.cpu().numpy(), xr.cpu().numpy()) <= 1e-10
+
+
+def test_1025x513_residual():
+    """The size the ring LU cannot hold: factorise at one shift and check the residual of the
+    solve with the exact block operator against eps |A|_F |x| / |b| (backward stability)."""
+    from pyeulerale_amd import _lib
+    from pyeulerale_amd import kernels as K
+    from pyeulerale_amd import workloads as W
+    from pyeulerale_amd.nested import Factorization
+    import ctypes
+    c = W.config("C5")
+    g, p = K.Grid(c.M, c.N), K.Params(aoa_deg=c.aoa)
+    X, V, U = storage(c.X), storage(c.V), storage(c.U[0])
+    JU = K.cs_linearize_odes(p, c.mach, g, X, V, U, want=("states",))[0]
+    s = 0.1j
+    say("1025x513 factorisation")
+    torch.cuda.synchronize()
+    import time
+    t0 = time.perf_counter()
+    f = Factorization(g, JU, s)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    rng = np.random.default_rng(6)
+    b = torch.as_tensor(crandn(rng, (2, 4, c.N, c.M))).cuda()
+    x = f.solve(b)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    say(f"1025x513: factor {t1 - t0:.2f} s, solve {t2 - t1:.3f} s, "
+        f"{f.singular_pivots} singular pivots")
+    sh = torch.full((2,), s, dtype=torch.complex128, device="cuda")
+    Ax = torch.empty_like(x)
+    _lib.call("pea_block_multi", g.M, g.N, 2, ctypes.c_void_p(JU.data_ptr()),
+              ctypes.c_void_p(sh.data_ptr()), 0, ctypes.c_void_p(x.data_ptr()),
+              ctypes.c_void_p(Ax.data_ptr()), K._stream())
+    res = torch.linalg.vector_norm((Ax - b).reshape(2, -1), dim=1)
+    bn = torch.linalg.vector_norm(b.reshape(2, -1), dim=1)
+    xn = torch.linalg.vector_norm(x.reshape(2, -1), dim=1)
+    JF = float(torch.linalg.vector_norm(JU)) + abs(s) * np.sqrt(4 * c.M * c.N)
+    for k in range(2):
+        r = float(res[k] / bn[k])
+        bound = 64 * np.finfo(float).eps * JF * float(xn[k] / bn[k])
+        say(f"1025x513 rhs {k}: residual {r:.2e}, backward-error scale {bound:.2e}")
+        assert r <= bound
