@@ -20,7 +20,9 @@ Also reported (same JSON line):
   workload against the complex-step paths;
 * second-order gradients (tangent-over-adjoint) at 513x257 against the reference's method,
   central differences of complex-step Jacobian products;
-* batched GMRES Arnoldi-step throughput for 32 shifts at 1025x513;
+* the 1025x513 steady state (PTC with exact nested-dissection solves) and its 32-shift
+  frequency response, batched against one exact solve per shift (time to solution, every
+  shift's residual checked);
 * 64 O-grids of uneven size in one multi-grid launch against one launch per grid, for the
   residual and for the whole step (residual, JVP and VJP of every grid, one graph);
 * the response example's steady state and 32-frequency response on the reference's 129x129 grid
@@ -59,7 +61,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-freq", action="store_true", help="skip the frequency-response solve")
     ap.add_argument("--no-multigrid", action="store_true", help="skip the 64-grid launch")
-    ap.add_argument("--no-gmres", action="store_true", help="skip the 1025x513 GMRES steps")
+    ap.add_argument("--gmres-steps", action="store_true",
+                    help="also time 30 unconverged Arnoldi steps of line-preconditioned batched "
+                         "GMRES at 1025x513 (step throughput only, not a solve)")
     ap.add_argument("--no-superlu", action="store_true",
                     help="skip timing the reference's host SuperLU solve")
     ap.add_argument("--profile", action="store_true",
@@ -72,6 +76,10 @@ def parse():
                          "a kernel trace of the Arnoldi steps")
     ap.add_argument("--only-multigrid", action="store_true",
                     help="only the 64-grid section (its JSON on one line), for a kernel trace")
+    ap.add_argument("--no-c5-freq", action="store_true",
+                    help="skip the 1025x513 steady state and 32-shift frequency response")
+    ap.add_argument("--only-c5-freq", action="store_true",
+                    help="only the 1025x513 frequency-response section (its JSON on one line)")
     ap.add_argument("--only-head", action="store_true",
                     help="only the headline residual launches (its JSON on one line), for a "
                          "kernel trace of the timed kernel")
@@ -560,6 +568,94 @@ def gmres_steps(torch, K, S: int = 32, steps: int = 30):
             "ms_per_shift_step_batched_matrix_free": t_mf / (S * steps) * 1e3}
 
 
+def freq_response_c5(torch, K, shifts: int = 32, groups: int = 8, budget: int = 60):
+    """The batched frequency response at 1025x513: PTC steady state (exact nested-dissection
+    solves, examples/response/run.py:89-123), then the pitch transfer functions at 32 shifts
+    s = i omega, omega = logspace(-2, 1, 32), right-hand side -(dR/dX + s dR/dV) dX/dalpha
+    (run.py:125-176).  Time to solution of all shifts:
+
+    * sequential: one exact nested-dissection factorisation and solve per shift (the order of
+      work of the reference's spsolve per shift, core.py:533-565);
+    * batched: the shifts grouped by |s| into ``groups`` groups, each solved together by GMRES
+      preconditioned by the exact factorisation of its middle shift; shifts GMRES has not
+      converged within ``budget`` Arnoldi steps (about the cost of one factorisation) are then
+      solved exactly.
+
+    Every shift's residual is checked with the exact block operator."""
+    import ctypes
+    from pyeulerale_amd import _lib, workloads as W
+    from pyeulerale_amd.core import SpatialDiscretization
+    from pyeulerale_amd.drivers import frequency_response, pseudo_transient_continuation
+    from pyeulerale_amd.nested import symbolic
+    c = W.config("C5")
+    t = time.perf_counter()
+    st = symbolic(c.M, c.N)
+    t_sym = time.perf_counter() - t
+    sd = SpatialDiscretization(vertices=c.X, angle_of_attack=0.0, mach_number=0.5,
+                               coefficient_length=1.0, coefficient_center=(0.25, 0.0))
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    hist = pseudo_transient_continuation(sd, rtol=1e-9, max_iter=30, solver="nested", log=None)
+    torch.cuda.synchronize()
+    t_ptc = time.perf_counter() - t
+    progress(f"1025x513 PTC: {len(hist)} iterations, {t_ptc:.1f} s")
+    sd.linearize()
+    omega = np.logspace(-2, 1, shifts)
+    speed = 0.5 * 1.4 ** 0.5
+    kred = omega * 0.5 / speed
+    g = sd._grid
+    JU = sd._get_blocks()["U"]
+    sh = torch.as_tensor(1j * omega, dtype=torch.complex128, device="cuda")
+
+    def residuals(dU, rhs):
+        x = dU.transpose(-1, -2).contiguous()
+        Ax = torch.empty_like(x)
+        _lib.call("pea_block_multi", g.M, g.N, len(omega), ctypes.c_void_p(JU.data_ptr()),
+                  ctypes.c_void_p(sh.data_ptr()), 0, ctypes.c_void_p(x.data_ptr()),
+                  ctypes.c_void_p(Ax.data_ptr()), K._stream())
+        b = rhs.transpose(-1, -2).reshape(len(omega), -1)
+        return (torch.linalg.vector_norm(Ax.reshape(len(omega), -1) + b, dim=1)
+                / torch.linalg.vector_norm(b, dim=1)).cpu().numpy()
+
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    _, cl_s, cm_s, dU_s, rhs = frequency_response(sd, kred, 1.0, 0.25, batched=False,
+                                                  solver="nested", log=None,
+                                                  return_states=True)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t
+    res_s = residuals(dU_s, rhs)
+    del dU_s
+    progress(f"1025x513 sequential: {t_seq:.1f} s")
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    _, cl_b, cm_b, dU_b, rhs = frequency_response(sd, kred, 1.0, 0.25, batched=True,
+                                                  solver="gmres", precond="nested",
+                                                  groups=groups, restart=budget,
+                                                  maxiter=budget, log=None, return_states=True)
+    torch.cuda.synchronize()
+    t_bat = time.perf_counter() - t
+    res_b = residuals(dU_b, rhs)
+    inf = sd.last_gmres
+    progress(f"1025x513 batched: {t_bat:.1f} s")
+    return {
+        "grid": "1025x513", "shifts": len(omega), "omega": [float(omega[0]), float(omega[-1])],
+        "symbolic_s": t_sym, "factor_bytes": st.factor_bytes(), "factor_flop": st.flops(),
+        "ptc": {"iterations": len(hist), "seconds": t_ptc, "rel_residual": hist[-1][2]},
+        "seconds_sequential_exact": t_seq, "seconds_batched": t_bat,
+        "speedup_batched_vs_sequential": t_seq / t_bat,
+        "batched_groups": groups, "gmres_budget": budget,
+        "gmres_steps": inf.iterations.tolist(),
+        "gmres_converged": int(inf.converged.sum()),
+        "solved_exactly_after_gmres": int((~inf.converged).sum()),
+        "max_residual_sequential": float(res_s.max()), "max_residual_batched": float(res_b.max()),
+        "max_rel_diff_cl": float(np.max(np.abs(cl_b - cl_s) / np.abs(cl_s))),
+        "max_rel_diff_cm": float(np.max(np.abs(cm_b - cm_s) / np.abs(cm_s))),
+        "cl": [[float(omega[i]), cl_s[i].real, cl_s[i].imag] for i in range(0, len(omega), 4)],
+        "cm": [[float(omega[i]), cm_s[i].real, cm_s[i].imag] for i in range(0, len(omega), 4)],
+    }
+
+
 def _cpu_model() -> str:
     try:
         for line in Path("/proc/cpuinfo").read_text().splitlines():
@@ -629,6 +725,9 @@ def main():
         return
     if args.only_freq:
         print(json.dumps(freq_response(torch, False, sequential=False)), flush=True)
+        return
+    if args.only_c5_freq:
+        print(json.dumps(freq_response_c5(torch, K)), flush=True)
         return
     # headline workload: NACA0004 1025x513, static grid, Ma 0.5, alpha 0, 8 states per launch
     mach, aoa = 0.5, 0.0
@@ -708,8 +807,8 @@ def main():
     if not args.no_large:
         extra["second_order_513x257"] = second_order(torch, K)
         progress("second-order products done")
-    if not args.no_gmres:
-        extra["gmres_1025x513"] = gmres_steps(torch, K)
+    if args.gmres_steps:
+        extra["gmres_step_throughput_1025x513"] = gmres_steps(torch, K)
         progress("gmres done")
     if not args.no_multigrid:
         extra["multigrid_64_grids"] = multigrid(torch, K)
@@ -717,6 +816,9 @@ def main():
     if not args.no_freq:
         extra["frequency_response_129x129"] = freq_response(torch, not args.no_superlu)
         progress("frequency response done")
+    if not args.no_c5_freq:
+        extra["frequency_response_1025x513"] = freq_response_c5(torch, K)
+        progress("1025x513 frequency response done")
     pmc = pmc_figures()
     line = {
         "metric": "residual evals/s + hand-written VJP vs complex-step JVP speedup (FP64, 1 GPU)",

@@ -300,7 +300,9 @@ int pea_dense_lu_solve(int S, int n, int nrhs, const double *A, const int *ipiv,
  *   adds the transposed block at the transposed position (the caller passes conj(s)).
  * pea_nd_factor: LU with partial pivoting of F11 (P x P, P = 4 Pc), F12 <- F11^-1 F12 and
  *   F22 <- F22 - F21 F12 (B = 4 Bc), in place for n fronts (rocSOLVER getrf/getrs, rocBLAS
- *   ZGEMM, strided batched); ipiv n x P, info n.
+ *   ZGEMM, strided batched); ipiv n x P, info n.  invert != 0: F11 is replaced by its inverse
+ *   (getri), Z = F11^-1 F12 goes to Zws (n x P x B column-major, F12 untouched) and the update
+ *   uses it (two GEMMs).
  * pea_nd_gather / pea_nd_scatter: out[k][r][4 q + v] <-> y[r ldy + v M N + cells[k(Pc+Bc) + off
  *   + q]] for q < cnt (gather writes 0 for padding; scatter skips it).
  * pea_nd_accumulate: y[r ldy + target[t]] -= sum_{e = ptr[t]}^{ptr[t+1]-1} W[(src_k[e] nrhs + r)
@@ -312,7 +314,8 @@ int pea_nd_assemble(int n, int Pc, int Bc, const int *cells, const int *child,
 int pea_nd_blocks(int ne, const int *entries, int node0, int M, int N, const double *JU,
                   double shift_re, double shift_im, int adjoint, int Pc, int Bc, double *F,
                   void *stream);
-int pea_nd_factor(int n, int P, int B, double *F, int *ipiv, int *info, void *stream);
+int pea_nd_factor(int n, int P, int B, double *F, int *ipiv, int *info, int invert, double *Zws,
+                  void *stream);
 int pea_nd_gather(int n, int cnt, int off, int Lc, const int *cells, int nrhs, int M, int N,
                   const double *y, long long ldy, double *out, void *stream);
 int pea_nd_scatter(int n, int cnt, int off, int Lc, const int *cells, int nrhs, int M, int N,

@@ -86,7 +86,7 @@ _SIGS = {
     "pea_nd_last_error": [],
     "pea_nd_assemble": [_I, _I, _I, _P, _P, _P, _I, _P, _P, _P],
     "pea_nd_blocks": [_I, _P, _I, _I, _I, _P, _D, _D, _I, _I, _I, _P, _P],
-    "pea_nd_factor": [_I, _I, _I, _P, _P, _P, _P],
+    "pea_nd_factor": [_I, _I, _I, _P, _P, _P, _I, _P, _P],
     "pea_nd_gather": [_I, _I, _I, _I, _P, _I, _I, _I, _P, _c.c_longlong, _P, _P],
     "pea_nd_scatter": [_I, _I, _I, _I, _P, _I, _I, _I, _P, _P, _c.c_longlong, _P],
     "pea_nd_accumulate": [_I, _I, _P, _P, _P, _P, _I, _P, _P, _c.c_longlong, _P],

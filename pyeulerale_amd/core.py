@@ -35,6 +35,13 @@ from .grid import read_plot3d
 F64, C128 = torch.float64, torch.complex128
 
 
+def _direct_kind(g) -> str:
+    """The exact solver for this grid: the ring block LU while one shift's factors take at most
+    16 GB (up to about 257x129 vertices), nested dissection beyond."""
+    from .ringlu import bytes_per_shift
+    return "ring" if bytes_per_shift(g.M, g.N) <= 16 * 10 ** 9 else "nested"
+
+
 def _as_tensor(a, dtype, device) -> torch.Tensor:
     if isinstance(a, torch.Tensor):
         return a.to(device=device, dtype=dtype)
@@ -400,8 +407,9 @@ class SpatialDiscretization:
                                   solver: str = "direct", **opts) -> torch.Tensor:
         """``(dR/dU - shift I)^-1 . d_odes`` at the linearization point (core.py:533-565).
 
-        ``solver``: ``"direct"`` (exact block LU over radial rings on the GPU; the factorisation
-        is kept for further solves with the same shift until the next ``linearize``),
+        ``solver``: ``"direct"`` (exact GPU factorisation, kept for further solves with the same
+        shift until the next ``linearize``: block LU over radial rings, ``"ring"``, on grids
+        where one shift's factors take at most 16 GB, nested dissection, ``"nested"``, beyond),
         ``"superlu"`` (the reference's sparse LU on the host) or ``"gmres"``."""
         return self._solve(d_odes, complex(shift), False, d_states, solver, opts)
 
@@ -417,11 +425,14 @@ class SpatialDiscretization:
         """All shifts at once: x_k = (dR/dU - s_k I)^-1 d_odes_k (or the adjoint).
         d_odes of shape (S, NUM_VAR, M, N); returns the same shape (views).
 
-        ``solver="direct"`` factorises all shifts together (batched ring LU, in chunks that fit
-        ``max_bytes`` of device memory).  ``solver="gmres"`` runs batched GMRES; shifts whose
-        residual is still above ``rtol`` after ``maxiter`` are solved again with the direct
-        method when ``fallback="direct"`` (the default), or left as they are (``fallback=None``;
-        ``last_gmres.converged`` tells which)."""
+        ``solver="direct"``: exact solves, the ring LU factorising all shifts together (in
+        chunks that fit ``max_bytes`` of device memory) where it fits, else one nested-dissection
+        factorisation per shift (``"ring"`` / ``"nested"`` choose).  ``solver="gmres"`` runs
+        batched GMRES; with ``precond="nested"`` the shifts are split into ``groups`` groups by
+        |s|, each preconditioned by the exact factorisation at its middle shift
+        (``last_groups``).  Shifts whose residual is still above ``rtol`` after ``maxiter`` are
+        solved again with the direct method when ``fallback="direct"`` (the default), or left as
+        they are (``fallback=None``; ``last_gmres.converged`` tells which)."""
         t = d_odes if isinstance(d_odes, torch.Tensor) else _as_tensor(d_odes, C128, self.device)
         shifts = np.asarray(shifts, dtype=complex).reshape(-1)
         S = len(shifts)
@@ -431,10 +442,34 @@ class SpatialDiscretization:
             raise ValueError(f"unknown fallback {fallback!r}")
         b = K.cells_in(t, C128, self.device)
         p, mach, g, X, V, U = self._point()
+        JU = self._get_blocks()["U"]
         if solver == "direct":
+            solver = _direct_kind(g)
+        if solver == "ring":
             from .ringlu import solve_shifts
-            x, self.last_residual = solve_shifts(g, self._get_blocks()["U"], b, shifts,
-                                                 adjoint=adjoint, **opts)
+            x, self.last_residual = solve_shifts(g, JU, b, shifts, adjoint=adjoint, **opts)
+        elif solver == "nested":
+            from .nested import solve_shifts_direct
+            x, self.last_residual = solve_shifts_direct(g, JU, b, shifts, adjoint=adjoint, **opts)
+        elif solver == "gmres" and opts.get("precond") == "nested":
+            from .gmres import GMRESInfo
+            from .nested import solve_shifts_anchored
+            opts = dict(opts)
+            opts.pop("precond")
+            x, groups = solve_shifts_anchored(g, JU, b, shifts, adjoint=adjoint,
+                                              point=(p, mach, g, X, V, U), **opts)
+            it = np.zeros(S, int)
+            res = np.zeros(S)
+            conv = np.zeros(S, bool)
+            for idx, _, inf in groups:
+                it[idx], res[idx], conv[idx] = inf.iterations, inf.residuals, inf.converged
+            self.last_gmres = GMRESInfo(iterations=it, residuals=res, converged=conv,
+                                        restarts=sum(inf.restarts for _, _, inf in groups),
+                                        seconds=sum(inf.seconds for _, _, inf in groups))
+            self.last_groups = groups
+            todo = np.flatnonzero(~conv)
+            if fallback == "direct" and todo.size:
+                x[todo], _ = self._direct_batch(g, JU, b[todo], shifts[todo], adjoint)
         elif solver == "gmres":
             from .gmres import BatchedGMRES
             gm = BatchedGMRES.from_discretization(self, shifts, adjoint=adjoint, **opts)
@@ -442,26 +477,39 @@ class SpatialDiscretization:
             self.last_gmres = gm.info
             todo = np.flatnonzero(~gm.info.converged)
             if fallback == "direct" and todo.size:
-                from .ringlu import solve_shifts
-                x[todo], _ = solve_shifts(g, self._get_blocks()["U"], b[todo], shifts[todo],
-                                          adjoint=adjoint)
+                x[todo], _ = self._direct_batch(g, JU, b[todo], shifts[todo], adjoint)
         else:
             raise ValueError(f"unknown solver {solver!r}")
         return x.transpose(-1, -2)
 
-    def _ring_factors(self, shift, adjoint):
-        from .ringlu import RingLU
-        key = (complex(shift), bool(adjoint))
+    @staticmethod
+    def _direct_batch(g, JU, b, shifts, adjoint):
+        if _direct_kind(g) == "ring":
+            from .ringlu import solve_shifts
+            return solve_shifts(g, JU, b, shifts, adjoint=adjoint)
+        from .nested import solve_shifts_direct
+        return solve_shifts_direct(g, JU, b, shifts, adjoint=adjoint)
+
+    def _direct_factors(self, shift, adjoint, kind):
+        key = (complex(shift), bool(adjoint), kind)
         if self._factors is None or self._factors[0] != key:
             self._factors = None        # release the previous factorisation first
             p, mach, g, X, V, U = self._point()
-            self._factors = (key, RingLU(g, self._get_blocks()["U"], [shift], adjoint=adjoint))
+            JU = self._get_blocks()["U"]
+            if kind == "ring":
+                from .ringlu import RingLU
+                fac = RingLU(g, JU, [shift], adjoint=adjoint)
+            else:
+                from .nested import Factorization
+                fac = Factorization(g, JU, shift, adjoint=adjoint)
+            self._factors = (key, fac)
         return self._factors[1]
 
     def _solve(self, rhs, shift, adjoint, out, solver, opts):
         b = self._cells(rhs, "rhs")
-        if solver == "direct":
-            lu = self._ring_factors(shift, adjoint)
+        if solver in ("direct", "ring", "nested"):
+            kind = _direct_kind(self._grid) if solver == "direct" else solver
+            lu = self._direct_factors(shift, adjoint, kind)
             x = lu.solve(b[None])[0]
             self.last_residual = lu.residual
         elif solver == "superlu":

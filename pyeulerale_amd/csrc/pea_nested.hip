@@ -13,7 +13,8 @@
 //   eliminated cell lies in the node's S, the shift on the diagonal blocks.  Each front entry
 //   receives at most one block, so the adds do not race.
 // * pea_nd_factor: partial-pivoting LU of F11 (rocSOLVER getrf), Z = F11^-1 F12 (getrs) and the
-//   update F22 - F21 Z (rocBLAS ZGEMM), strided batched over the level's fronts.
+//   update F22 - F21 Z (rocBLAS ZGEMM), strided batched over the level's fronts; or F11 inverted
+//   from its LU (getri) and Z and the update as two GEMMs, so that solves are GEMMs as well.
 // * k_nd_gather / k_nd_scatter / k_nd_accumulate: the solve's moves between the global vectors
 //   (v * C + cell layout) and the fronts' S or B parts; the subtraction of the level's
 //   contributions to shared ring cells sums them per target in a fixed order (deterministic).
@@ -213,21 +214,39 @@ int pea_nd_blocks(int ne, const int* entries, int node0, int M, int N, const dou
     return hipGetLastError() == hipSuccess ? PEA_OK : nfail(PEA_ERR_HIP, "k_nd_blocks");
 }
 
-int pea_nd_factor(int n, int P, int B, double* F, int* ipiv, int* info, void* stream) {
+int pea_nd_factor(int n, int P, int B, double* F, int* ipiv, int* info, int invert, double* Zws,
+                  void* stream) {
     if (n < 1 || P < 1 || B < 0) return nfail(PEA_ERR_DIMS, "bad dimensions");
-    if (!F || !ipiv || !info) return nfail(PEA_ERR_NULL, "null argument");
+    if (!F || !ipiv || !info || (invert && B > 0 && !Zws)) return nfail(PEA_ERR_NULL, "null argument");
     rocblas_handle h = nd_handle((hipStream_t)stream);
     if (!h) return nfail(PEA_ERR_HIP, "rocBLAS handle");
     const rocblas_int L = P + B;
     const rocblas_stride sF = (rocblas_stride)L * L;
     auto* A = reinterpret_cast<rocblas_double_complex*>(F);
+    const rocblas_double_complex minus_one(-1.0, 0.0), one(1.0, 0.0), zero(0.0, 0.0);
     rocblas_status st = rocsolver_zgetrf_strided_batched(h, P, P, A, L, sF, ipiv, P, info, n);
     if (st != rocblas_status_success) return nfail(PEA_ERR_HIP, "rocsolver getrf (%d)", (int)st);
+    if (invert) {
+        // F11 <- F11^-1, then Z = F11^-1 F12 and the update are GEMMs
+        st = rocsolver_zgetri_strided_batched(h, P, A, L, sF, ipiv, P, info, n);
+        if (st != rocblas_status_success) return nfail(PEA_ERR_HIP, "rocsolver getri (%d)", (int)st);
+        if (B == 0) return PEA_OK;
+        auto* Z = reinterpret_cast<rocblas_double_complex*>(Zws);
+        const rocblas_stride sZ = (rocblas_stride)P * B;
+        st = rocblas_zgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, P, B,
+                                           P, &one, A, L, sF, A + (long long)P * L, L, sF, &zero,
+                                           Z, P, sZ, n);
+        if (st != rocblas_status_success) return nfail(PEA_ERR_HIP, "rocblas zgemm (%d)", (int)st);
+        st = rocblas_zgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, B, B,
+                                           P, &minus_one, A + P, L, sF, Z, P, sZ, &one,
+                                           A + P + (long long)P * L, L, sF, n);
+        return st == rocblas_status_success ? PEA_OK
+                                            : nfail(PEA_ERR_HIP, "rocblas zgemm (%d)", (int)st);
+    }
     if (B == 0) return PEA_OK;
     st = rocsolver_zgetrs_strided_batched(h, rocblas_operation_none, P, B, A, L, sF, ipiv, P,
                                           A + (long long)P * L, L, sF, n);
     if (st != rocblas_status_success) return nfail(PEA_ERR_HIP, "rocsolver getrs (%d)", (int)st);
-    const rocblas_double_complex minus_one(-1.0, 0.0), one(1.0, 0.0);
     st = rocblas_zgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, B, B, P,
                                        &minus_one, A + P, L, sF, A + (long long)P * L, L, sF, &one,
                                        A + P + (long long)P * L, L, sF, n);

@@ -92,12 +92,14 @@ def export_pressure_coefficients(file_name, vertices, pressure_coef, pressure_co
 
 def frequency_response(sd, reduced_frequencies, chord: float, axis_location: float,
                        batched: bool = True, solver: str = "direct",
-                       log: Callable | None = print, cp_file=None, **solve_opts):
+                       log: Callable | None = print, cp_file=None, return_states: bool = False,
+                       **solve_opts):
     """Transfer functions from pitch angle to c_l and c_m at the linearization point.
 
     ``sd`` must be linearized at a steady state and constructed with
     ``coefficient_length=chord`` and ``coefficient_center=(axis_location, 0)``.
-    Returns (reduced frequencies, L c_l / L alpha, L c_m / L alpha) as complex arrays.
+    Returns (reduced frequencies, L c_l / L alpha, L c_m / L alpha) as complex arrays, and with
+    ``return_states`` also the state responses (S, 4, M, N) and the right-hand sides solved for.
     With ``cp_file`` the steady c_p and its transfer at the last frequency are exported
     (:func:`export_pressure_coefficients`), as the example does.
 
@@ -138,6 +140,8 @@ def frequency_response(sd, reduced_frequencies, chord: float, axis_location: flo
             f"{'red frequency':>15} {'lift coefficient':>25} {'moment coefficient':>25}")
         for i in range(len(k)):
             log(f"{k[i]:>15.3e} {cl[i]:>+25.3e} {cm[i]:>+25.3e}")
+    if return_states:
+        return k, cl, cm, dU, rhs
     return k, cl, cm
 
 

@@ -8,7 +8,9 @@ The reference solves one shift at a time with SuperLU on a freshly assembled CSR
   hand-written tangent (adjoint for the transposed system) of all S vectors in one batched
   launch (``pea_jvp_odes_multi`` / ``pea_vjp_odes_states_multi``), no assembled matrix;
 * right preconditioning per shift: ``"line"`` (block LU of every radial line,
-  ``pea_line_factor``/``pea_line_solve``), ``"jacobi"`` (4x4 diagonal blocks) or ``None``;
+  ``pea_line_factor``/``pea_line_solve``), ``"jacobi"`` (4x4 diagonal blocks), ``"nested"`` (the
+  exact nested-dissection factorisation of one anchor shift for every shift of the batch, all
+  shifts in one multi-right-hand-side solve) or ``None``;
 * orthogonalisation: classical Gram-Schmidt with selective reorthogonalisation (the projection is
   repeated only where it removed more than half of the vector's norm squared), in two or three
   fused passes over the basis;
@@ -56,7 +58,7 @@ class BatchedGMRES:
     def __init__(self, g: K.Grid, JU: torch.Tensor | None, shifts, *, adjoint: bool = False,
                  point=None, precond: str | None = "line", operator: str = "blocks",
                  restart: int = 30, maxiter: int = 600, rtol: float = 1e-10,
-                 graphs: bool = True, check_every: int = 10):
+                 graphs: bool = True, check_every: int = 10, factor=None):
         self.g = g
         self.adjoint = bool(adjoint)
         self.shifts = np.asarray(shifts, dtype=complex).reshape(-1)
@@ -78,7 +80,7 @@ class BatchedGMRES:
             raise ValueError("operator 'blocks' needs the assembled blocks")
         if operator == "jvp" and point is None:
             raise ValueError("operator 'jvp' needs the linearization point")
-        if precond not in ("line", "jacobi", None):
+        if precond not in ("line", "jacobi", "nested", None):
             raise ValueError(f"unknown preconditioner {precond!r}")
         if precond is not None and JU is None:
             raise ValueError("preconditioning needs the assembled blocks")
@@ -86,7 +88,19 @@ class BatchedGMRES:
         self.F = None
         M, N, S = g.M, g.N, self.S
         st = K._stream()
-        if precond == "line":
+        if precond == "nested":
+            # one exact factorisation at an anchor shift preconditions every shift of the batch
+            # (their systems differ from the anchor's by (s_anchor - s_k) I); its solves run
+            # library calls that are not captured, so the Arnoldi steps run eagerly
+            if factor is None:
+                from .nested import Factorization
+                anchor = self.shifts[len(self.shifts) // 2]
+                factor = Factorization(g, JU, anchor, adjoint=self.adjoint)
+            if factor.adjoint != self.adjoint:
+                raise ValueError("the factorisation is of the other system")
+            self.F = factor
+            self.graphs = False
+        elif precond == "line":
             nbytes = int(_lib.load().pea_line_factor_bytes(M, N, S))
             self.F = torch.empty(nbytes // 8, dtype=torch.float64, device=dev)
             _lib.call("pea_line_factor", M, N, S, _p(JU), _p(self.sh), int(self.adjoint),
@@ -127,6 +141,8 @@ class BatchedGMRES:
                       int(self.adjoint), _p(r), _p(z), K._stream())
         elif self.precond == "jacobi":
             _lib.call("pea_jacobi_solve", g.M, g.N, self.S, _p(self.F), _p(r), _p(z), K._stream())
+        elif self.precond == "nested":
+            self.F.precondition(r, z)
         else:
             z.copy_(r)
         return z

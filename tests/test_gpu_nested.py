@@ -31,7 +31,8 @@ def storage(a):
 
 @pytest.mark.parametrize("M,N,leaf", [(8, 16, 16), (12, 26, 24), (16, 32, 64)])
 @pytest.mark.parametrize("adjoint", [False, True])
-def test_random_blocks_against_dense_solve(M, N, leaf, adjoint):
+@pytest.mark.parametrize("inverse", [True, False])
+def test_random_blocks_against_dense_solve(M, N, leaf, adjoint, inverse):
     from pyeulerale_amd import kernels as K
     from pyeulerale_amd.nested import Factorization
     say(f"random blocks M={M} N={N} leaf={leaf} adjoint={adjoint}")
@@ -44,9 +45,10 @@ def test_random_blocks_against_dense_solve(M, N, leaf, adjoint):
     b = crandn(rng, (3, 4 * M * N))
     ref = np.linalg.solve(A, b.T).T
     f = Factorization(K.Grid(M, N), torch.as_tensor(JU).cuda(), shift, adjoint=adjoint,
-                      leaf=leaf)
+                      leaf=leaf, inverse=inverse)
     assert f.singular_pivots == 0
-    x = f.solve(torch.as_tensor(b.reshape(3, 4, N, M)).cuda()).cpu().numpy().reshape(3, -1)
+    x = f.solve(torch.as_tensor(b.reshape(3, 4, N, M)).cuda(), refine=0)
+    x = x.cpu().numpy().reshape(3, -1)
     assert rel(x, ref) <= 1e-10, rel(x, ref)
 
 
@@ -138,3 +140,53 @@ def test_1025x513_residual():
         bound = 64 * np.finfo(float).eps * JF * float(xn[k] / bn[k])
         say(f"1025x513 rhs {k}: residual {r:.2e}, backward-error scale {bound:.2e}")
         assert r <= bound
+
+
+def test_1025x513_frequency_response_batched_equals_sequential():
+    """The 1025x513 configuration end to end: PTC steady state with exact nested-dissection
+    solves (examples/response/run.py:89-123), then the pitch transfer functions at 8 of the 32
+    frequencies (every fourth of omega = logspace(-2, 1, 32)) solved batched (GMRES grouped
+    around anchor factorisations, exact solves where GMRES does not converge within 60 steps)
+    and one exact factorisation per shift; every shift's residual with the exact operator
+    <= 1e-10 and c_l, c_m equal to 1e-8."""
+    from pyeulerale_amd import workloads as W
+    from pyeulerale_amd.core import SpatialDiscretization
+    from pyeulerale_amd.drivers import frequency_response, pseudo_transient_continuation
+    c = W.config("C5")
+    sd = SpatialDiscretization(vertices=c.X, angle_of_attack=0.0, mach_number=0.5,
+                               coefficient_length=1.0, coefficient_center=(0.25, 0.0))
+    hist = pseudo_transient_continuation(sd, rtol=1e-9, max_iter=30, solver="nested",
+                                         log=lambda m: say(m.strip()))
+    assert hist[-1][2] <= 1e-9
+    sd.linearize()
+    omega = np.logspace(-2, 1, 32)[::4]
+    kred = omega * 0.5 / (0.5 * 1.4 ** 0.5)
+    _, cl_s, cm_s = frequency_response(sd, kred, 1.0, 0.25, batched=False, solver="nested",
+                                       log=None)
+    _, cl_b, cm_b, dU, rhs = frequency_response(sd, kred, 1.0, 0.25, batched=True,
+                                                solver="gmres", precond="nested", groups=4,
+                                                restart=60, maxiter=60, log=None,
+                                                return_states=True)
+    say(f"1025x513 response: GMRES steps {sd.last_gmres.iterations.tolist()}, "
+        f"converged {sd.last_gmres.converged.tolist()}")
+    # residual of every shift with the exact block operator: (J - s I) dU = -rhs
+    import ctypes
+    from pyeulerale_amd import _lib
+    from pyeulerale_amd import kernels as K
+    g = sd._grid
+    shifts = torch.as_tensor(1j * kred * (0.5 * 1.4 ** 0.5) / 0.5, dtype=torch.complex128,
+                             device="cuda")
+    x = dU.transpose(-1, -2).contiguous()
+    Ax = torch.empty_like(x)
+    _lib.call("pea_block_multi", g.M, g.N, len(kred), ctypes.c_void_p(sd._get_blocks()["U"].data_ptr()),
+              ctypes.c_void_p(shifts.data_ptr()), 0, ctypes.c_void_p(x.data_ptr()),
+              ctypes.c_void_p(Ax.data_ptr()), K._stream())
+    b = rhs.transpose(-1, -2).reshape(len(kred), -1)
+    res = (torch.linalg.vector_norm((Ax.reshape(len(kred), -1) + b), dim=1)
+           / torch.linalg.vector_norm(b, dim=1)).cpu().numpy()
+    say(f"1025x513 response: residuals {np.array2string(res, precision=1)}")
+    assert (res <= 1e-10).all()
+    e_l = np.max(np.abs(cl_b - cl_s) / np.abs(cl_s))
+    e_m = np.max(np.abs(cm_b - cm_s) / np.abs(cm_s))
+    say(f"1025x513 response: batched vs sequential c_l {e_l:.1e}, c_m {e_m:.1e}")
+    assert e_l <= 1e-8 and e_m <= 1e-8
