@@ -97,6 +97,14 @@ int pea_multi_launch(int G, int total_tiles, int tangent, const void *workspace,
 int pea_compute_odes_multi(int G, const pea_grid *grids, void *workspace, void *stream);
 int pea_jvp_odes_multigrid(int G, const pea_grid *grids, void *workspace, void *stream);
 
+/* Counts, on the device, the cells whose state is non-finite or non-physical (rho <= 0 or
+ * p <= 0) into status[0] and, when R is not NULL, the cells with a non-finite residual into
+ * status[1] (status: 2 ints on the device, overwritten).  The reference evaluates such states in
+ * complex arithmetic and returns an imaginary wave speed (f90:142-148); here the residual of
+ * every cell whose faces use such a state is NaN and this call reports it without a host
+ * round trip. */
+int pea_check_states(int M, int N, const double *U, const double *R, int *status, void *stream);
+
 /* Surface forces F_n = p(U_{0,n}) normal(X_{0,n} -> X_{0,n+1}).  Replaces compute_forces
  * (f90:1156-1174).  If coef != NULL (device, 3 doubles) it also receives the reductions
  * (sum F_x, sum F_z, sum (z_p - zc) F_x - (x_p - xc) F_z) with x_p the face midpoints (the sums

@@ -41,6 +41,7 @@ _SIGS = {
     "pea_compute_odes": [_P, _D, _I, _I, _P, _P, _P, _P, _P],
     "pea_compute_odes_batch": [_P, _D, _I, _I, _I, _P, _P, _P, _P, _P],
     "pea_compute_forces": [_I, _I, _P, _P, _P, _D, _D, _P, _P],
+    "pea_check_states": [_I, _I, _P, _P, _P, _P],
     "pea_jvp_odes": [_P, _D, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "pea_jvp_odes_multi": [_P, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "pea_vjp_odes_states_multi": [_P, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P],

@@ -225,6 +225,14 @@ class SpatialDiscretization:
         K.compute_odes(self._params(), self._mach, self._grid, self._X, self._V, self._U,
                        self._R)
 
+    def check_states(self) -> dict:
+        """Counts the cells with a non-finite or non-physical state (rho <= 0 or p <= 0) and the
+        cells whose last computed residual is not finite.  The reference evaluates such states
+        in complex arithmetic (an imaginary wave speed, spatial_discretization.f90:142-148); in
+        FP64 they make the residual NaN wherever their faces enter, and this reports it."""
+        st = K.check_states(self._grid, self._U, self._R).cpu().numpy()
+        return {"nonphysical_states": int(st[0]), "nonfinite_odes": int(st[1])}
+
     def compute_forces(self) -> None:
         """Computes ``forces`` from ``states`` and ``vertices`` (core.py:601-607)."""
         K.compute_forces(self._grid, self._X, self._U, self._F)

@@ -291,10 +291,14 @@ PEA_HD T pressure_q(const T* u) {
     return (kGamma - 1.0) * (u[3] - qdiv(u[1] * u[1] + u[2] * u[2], u[0]) / 2.0);
 }
 
-// Per-cell pressure and sound speed sqrt(gamma p / rho) (f90:128-133, 147).
+// Per-cell pressure and sound speed sqrt(gamma p / rho) (f90:128-133, 147).  A state that is
+// not finite or not physical (rho <= 0 or p <= 0) gets a NaN pressure, so that the residual of
+// every cell whose faces use it is NaN: the reference's complex arithmetic returns finite values
+// there (an imaginary sound speed that the biased maximum of the dissipation may discard).
 template <class T>
 PEA_HD void cell_pc(const T* u, T& p, T& c) {
     p = pressure_q(u);
+    if (!(val(u[0]) > 0.0 && val(p) > 0.0)) p = p * __builtin_nan("");
     c = ssqrt(qdiv(kGamma * p, u[0]));
 }
 

@@ -1971,6 +1971,36 @@ int launch_vjp(const pea_params& P, const Ld& ld, const double* w, double* gU, d
 
 }  // namespace
 
+// =============================================================================== state checks
+// Non-physical states: the reference evaluates p < 0 or rho < 0 in complex arithmetic, where the
+// wave speed sqrt(gamma p / rho) becomes imaginary (f90:142-148); in FP64 it is NaN, and NaN or
+// Inf in a state reaches the residual of every cell whose faces use it.  This pass counts the
+// cells with a non-finite or non-physical state (rho <= 0 or p <= 0) into status[0] and those
+// with a non-finite residual into status[1] (one atomic per wavefront, only where a lane counts).
+__global__ void k_zero_ints(int* __restrict__ p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+__global__ void __launch_bounds__(256)
+k_check_states(Grid g, const double* __restrict__ U, const double* __restrict__ R,
+               int* __restrict__ status) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    bool bad_u = false, bad_r = false;
+    if (c < g.C) {
+        const double rho = U[c], jx = U[g.C + c], jz = U[2L * g.C + c], e = U[3L * g.C + c];
+        const double p = (kGamma - 1.0) * (e - (jx * jx + jz * jz) / rho / 2.0);
+        bad_u = !(isfinite(rho) && isfinite(jx) && isfinite(jz) && isfinite(e)) ||
+                !(rho > 0.0) || !(p > 0.0);
+        if (R)
+            for (int v = 0; v < 4; ++v) bad_r = bad_r || !isfinite(R[(long)v * g.C + c]);
+    }
+    const unsigned long long bu = __ballot(bad_u), br = __ballot(bad_r);
+    if ((threadIdx.x & 63) == 0) {
+        if (bu) atomicAdd(&status[0], (int)__popcll(bu));
+        if (br) atomicAdd(&status[1], (int)__popcll(br));
+    }
+}
+
 // =============================================================================== C ABI
 extern "C" {
 
@@ -2106,6 +2136,18 @@ int pea_jvp_odes_multigrid(int G, const pea_grid* grids, void* workspace, void* 
     int total = 0;
     if (int rc = pea_multi_prepare(G, grids, 1, workspace, &total, stream)) return rc;
     return pea_multi_launch(G, total, 1, workspace, stream);
+}
+
+int pea_check_states(int M, int N, const double* U, const double* R, int* status,
+                     void* stream) {
+    if (int rc = check_dims(M, N)) return rc;
+    if (!U || !status) return fail(PEA_ERR_NULL, "null argument");
+    const Grid g = make_grid(M, N);
+    k_zero_ints<<<1, 64, 0, as_stream(stream)>>>(status, 2);
+    PEA_LAUNCH_CHECK("k_zero_ints");
+    k_check_states<<<blocks_for(g.C, 256), 256, 0, as_stream(stream)>>>(g, U, R, status);
+    PEA_LAUNCH_CHECK("k_check_states");
+    return PEA_OK;
 }
 
 int pea_compute_forces(int M, int N, const double* X, const double* U, double* F, double xc,

@@ -202,6 +202,16 @@ def vjp_odes_multigrid(items):
     return MultiGrid(items, adjoint=True).run()
 
 
+def check_states(g: Grid, U, R=None, status=None) -> torch.Tensor:
+    """Device int32 tensor (2,): cells with a non-finite or non-physical state (rho <= 0 or
+    p <= 0), and cells with a non-finite residual R (when given).  Asynchronous."""
+    _need(U, g.cell_shape, F64, "states")
+    _opt(R, g.cell_shape, F64, "odes")
+    status = torch.empty(2, dtype=torch.int32, device=U.device) if status is None else status
+    _lib.call("pea_check_states", g.M, g.N, _ptr(U), _ptr(R), _ptr(status), _stream())
+    return status
+
+
 def compute_forces(g: Grid, X, U, F=None, center=(0.0, 0.0), coef=None):
     _need(X, g.vert_shape, F64, "vertices")
     _need(U, g.cell_shape, F64, "states")

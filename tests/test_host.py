@@ -186,3 +186,23 @@ def test_noalloc_adjoint_refuses_grids_beyond_its_scratch():
     rc = lib.pea_vjp_odes_noalloc(ctypes.byref(p), 0.5, 8, 8, None, d, d, d, None, None, None,
                                   None, None)
     assert rc == -2
+
+
+def test_f2py_compatible_module_has_the_reference_procedures():
+    """The 20 procedures and 4 module variables of spatial_discretization.f90:63-87, each with
+    f2py's visible argument count (the array dimensions are hidden, e.g. f90:498-500)."""
+    import inspect
+    from pyeulerale_amd.f2py_compat import PROCEDURES, spatial_discretization as disc
+    arity = {"set_free_stream_state": 2, "convert_odes_wrt_states": 4, "compute_forces": 3,
+             "compute_forces_wrt_states": 3, "compute_forces_wrt_vertices": 3}
+    assert len(PROCEDURES) == 20 == len(set(PROCEDURES))
+    for name in PROCEDURES:
+        fn = getattr(disc, name)
+        want = arity.get(name, 5 if name.startswith("compute_odes") else 3)
+        assert len(inspect.signature(fn).parameters) == want, name
+    for var, default in (("aoa", 0.0), ("jst_k2", 1.0), ("jst_k4", 1 / 32), ("jst_c4", 2.0)):
+        a = getattr(disc, var)
+        assert a.shape == () and float(a) == default
+    disc.aoa[...] = 1.5                    # as core.py:258-263 writes them
+    assert disc._params().aoa_deg == 1.5
+    disc.aoa[...] = 0.0
