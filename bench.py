@@ -170,6 +170,7 @@ def products(torch, K, A, Rv, thickness, mach, aoa, alpha, alpha_dot, far, reps)
     gU = torch.empty_like(dU)
     JU, _, _, _ = K.cs_linearize_odes(p, mach, g, Xd, Vd, Ud, want=("states",))
     tile = K.autotune_vjp(p, mach, g, Xd, Vd, Ud)
+    tuned1 = K.autotune_face(p, mach, g, Xd, Vd, Ud, kinds=("jvp", "vjp"), reps=5, batch=1)
     t = {}
     t["jvp"] = graph_time_ms(torch, lambda: K.jvp_odes(p, mach, g, Xd, Vd, Ud, dU=dU, dR=out),
                              reps)
@@ -185,6 +186,7 @@ def products(torch, K, A, Rv, thickness, mach, aoa, alpha, alpha_dot, far, reps)
     t["residual"] = graph_time_ms(torch, lambda: K.compute_odes(p, mach, g, Xd, Vd, Ud), reps)
     # 32 products per launch (one per grid z): time per product
     kb = 32
+    tuned32 = K.autotune_face(p, mach, g, Xd, Vd, Ud, kinds=("jvp", "vjp"), reps=2, batch=kb)
     db = torch.randn((kb, *g.cell_shape), dtype=torch.complex128, device=Ud.device,
                      generator=torch.Generator(device=Ud.device).manual_seed(2))
     ob = torch.empty_like(db)
@@ -207,6 +209,8 @@ def products(torch, K, A, Rv, thickness, mach, aoa, alpha, alpha_dot, far, reps)
         "rel_err_jvp_vs_cs_blocks": rel(jvp, csd), "rel_err_vjp_vs_cs_blocks": rel(vjp, asm),
         "rel_err_jvp_vs_cs_dir": rel(jvp, K.cs_jvp_odes(p, mach, g, Xd, Vd, Ud, dU=dU)),
         "vjp_tile": tile,
+        "face_tiles_autotuned": {"one_per_launch": {k: v[0] for k, v in tuned1.items()},
+                                 "32_per_launch": {k: v[0] for k, v in tuned32.items()}},
         "speedup_vjp_vs_cs_dir": t["cs_dir"] / t["vjp"],
         "speedup_jvp_vs_cs_dir": t["cs_dir"] / t["jvp"],
         "speedup_vjp_vs_cs_asm_rev": t["cs_asm_apply_rev"] / t["vjp"],
@@ -478,6 +482,13 @@ def multigrid(torch, K, G: int = 64, seed: int = 0, reps: int = 10):
                                   amp * 0.2 * np.cos(0.4), 20.0, 1, seed=int(rng.integers(1 << 30)))
         items.append((K.Params(aoa_deg=aoa), mach, dev(X), dev(V), dev(U[0])))
         cells += M * N
+    # the project's autotuner picks the multi-grid launches' tile shapes for this family
+    crand0 = lambda U: torch.randn(U.shape, dtype=torch.complex128, device="cuda")
+    tuned = K.autotune_multigrid([it + (crand0(it[4]),) for it in items], reps=3)
+    # the one-launch-per-grid comparison uses the same shapes (bitwise equal results)
+    for kind, name in enumerate(("residual", "jvp", "vjp")):
+        for it in items:
+            K.set_face_tile(kind, it[4].shape[2], it[4].shape[1], tuned[name][0])
     mg = K.MultiGrid(items)
     singles = [(p, mach, K.Grid(U.shape[2], U.shape[1]), X, V, U, torch.empty_like(U))
                for p, mach, X, V, U in items]
@@ -522,6 +533,8 @@ def multigrid(torch, K, G: int = 64, seed: int = 0, reps: int = 10):
                  for (R, dR), j in zip(mgt.outputs, jv))
     same_a = all(torch.equal(gU, a[7]) for gU, a in zip(mga.outputs, adj))
     return {"grids": G, "cells": cells, "tiles": mg.total_tiles,
+            "autotuned_tiles": {k: {"tile": v[0], "ms": {str(t): ms for t, ms in v[1].items()}}
+                                for k, v in tuned.items()},
             "ms_multi_launch": t_multi, "ms_one_launch_per_grid": t_single,
             "speedup": t_single / t_multi, "cells_per_s_multi": cells / (t_multi * 1e-3),
             "bitwise_equal": bool(same),
@@ -743,6 +756,10 @@ def main():
     Ud = torch.as_tensor(np.ascontiguousarray(np.swapaxes(U, -1, -2))).cuda()
     Rd = torch.empty_like(Ud)
     step = lambda: K.compute_odes_batch(p, mach, g, Xd, Vd, Ud, Rd)
+    # the project's autotuner picks the residual kernel's tile shape for this grid and batch
+    tuned_head = K.autotune_face(p, mach, g, Xd, Vd, Ud[0].contiguous(), kinds=("residual",),
+                                 reps=5, batch=HEAD_BATCH)["residual"]
+    head_tile = K.face_tile(0, g.M, g.N)
     # The K timed launches are replayed from one HIP graph, and so are the warm-up launches after
     # the first, which runs eagerly so that nothing is loaded during capture.  The events bracket
     # the timed replay on the stream the kernels run on.
@@ -832,7 +849,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": pmc.get("traffic_bytes_per_launch"),
-                     "kernel": "k_face_fwd<double, double, 16, 16, LdReal, StoreRealB>",
+                     "kernel": f"k_face_fwd<double, double, {head_tile[1]}, {head_tile[2]}, "
+                               f"LdReal, StoreRealB>",
+                     "tile_autotuned_ms": {str(k): v for k, v in tuned_head[1].items()},
                      "kernel_ms": kern_ms, "algorithmic_bytes_per_launch": bytes_launch,
                      "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE of the 8-state "
                                        "1025x513 launch, profiles/pmc_face_kernels_1025x513.md",

@@ -62,6 +62,20 @@ int pea_num_tiles(void);
 int pea_set_tile(int M, int N, int tile);
 int pea_get_tile(int M, int N);
 
+/* Tile shapes of the face-based kernels (no counterpart in the reference: the autotuner's
+ * control).  kind 0: residual (pea_compute_odes, pea_compute_odes_batch), 1: states tangent
+ * (pea_jvp_odes along dU only, pea_jvp_odes_multi), 2: states adjoint (pea_vjp_odes to the
+ * states only, pea_vjp_odes_states_multi); kinds 3-5: the same kernels in multi-grid launches
+ * (pea_multi_prepare modes 0-2; process-wide, M = N = 0).  pea_num_face_tiles() shapes exist
+ * per kind; pea_set_face_tile registers one for grids of M x N cells (M = N = 0: every grid;
+ * tile = -1 clears); pea_get_face_tile returns the shape in use (and its cells per tile in *tm,
+ * *tn when not NULL), -1 when no adjoint tile fits.  The shapes agree to round-off, not bitwise:
+ * a face on a tile's edge is evaluated by other instructions than an interior face.  A
+ * multi-grid launch is bitwise equal to single-grid calls with the same shape. */
+int pea_num_face_tiles(void);
+int pea_set_face_tile(int kind, int M, int N, int tile);
+int pea_get_face_tile(int kind, int M, int N, int *tm, int *tn);
+
 /* ------------------------------------------------------------------ primal */
 
 /* Fill U with the free-stream state.  Replaces set_free_stream_state (f90:471-482). */
@@ -86,9 +100,11 @@ int pea_compute_odes_batch(const pea_params *prm, double mach, int M, int N, int
  *   tile count; it synchronises the stream.  tangent = 0: residuals (R written); tangent = 1:
  *   states-direction JVP (pea_jvp_odes with dU only; dR written, R too when non-NULL);
  *   tangent = 2: states adjoint (pea_vjp_odes to the states only), dU holding the seed w and
- *   dR receiving (dR/dU)^T w; every grid needs N >= 18 (the adjoint tile's width + 3).
- * pea_multi_launch: one launch over the prepared table; it allocates and copies nothing, so it
- *   may be captured in a HIP graph and replayed while the table and arrays stay valid.
+ *   dR receiving (dR/dU)^T w; every grid needs N >= the adjoint tile's width + 3 (18 for the
+ *   default tile).
+ * pea_multi_launch: one launch over the prepared table with the tile shape pea_multi_prepare
+ *   chose for it (pea_set_face_tile kinds 3-5); it allocates and copies nothing, so it may be
+ *   captured in a HIP graph and replayed while the table and arrays stay valid.
  * pea_compute_odes_multi / pea_jvp_odes_multigrid: prepare + launch. */
 long long pea_multi_workspace_bytes(int G);
 int pea_multi_prepare(int G, const pea_grid *grids, int tangent, void *workspace,

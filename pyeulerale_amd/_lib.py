@@ -37,6 +37,9 @@ _SIGS = {
     "pea_set_tile": [_I, _I, _I],
     "pea_get_tile": [_I, _I],
     "pea_num_tiles": [],
+    "pea_num_face_tiles": [],
+    "pea_set_face_tile": [_I, _I, _I, _I],
+    "pea_get_face_tile": [_I, _I, _I, _P, _P],
     "pea_set_free_stream_state": [_P, _D, _I, _I, _P, _P],
     "pea_compute_odes": [_P, _D, _I, _I, _P, _P, _P, _P, _P],
     "pea_compute_odes_batch": [_P, _D, _I, _I, _I, _P, _P, _P, _P, _P],

@@ -1756,19 +1756,11 @@ unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 // cell kernels: smaller blocks when the grid is small so that all CUs get work
 int cell_block(long C) { return C >= 256L * 1024 ? 256 : 64; }
 
-// tiles of the face-based forward kernels (m fastest within a tile)
-// Tile shapes of the forward kernels (cells per workgroup, one thread per cell); the defaults
-// are the fastest measured on MI355X and may be overridden at build time.
-#ifndef PEA_FWD_TILE_M
-#define PEA_FWD_TILE_M 16
-#define PEA_FWD_TILE_N 16
-#endif
-#ifndef PEA_JVP_TILE_M
-#define PEA_JVP_TILE_M 16
-#define PEA_JVP_TILE_N 16
-#endif
-constexpr int kFwdTM = PEA_FWD_TILE_M, kFwdTN = PEA_FWD_TILE_N;
-constexpr int kJvpTM = PEA_JVP_TILE_M, kJvpTN = PEA_JVP_TILE_N;
+// Tile of the face-based forward kernels (cells per workgroup, one thread per cell, m fastest
+// within a tile) for the tangent along several inputs; the residual and the states tangent
+// choose among several shapes at run time (kFaceTiles below).
+constexpr int kFwdTM = 16, kFwdTN = 16;
+constexpr int kJvpTM = 16, kJvpTN = 16;
 
 // ------------------------------------------------------------------ many grids, one launch
 // Descriptor of one grid of a multi-grid launch as the kernel reads it: the public pea_grid
@@ -1785,8 +1777,8 @@ struct GridDesc {
 // read through the same L2.  gridDim.x is the total tile count; a workgroup finds its grid by
 // binary search over the first-tile table.  (A persistent loop over tiles would keep every
 // tile-invariant address and constant live across the loop.)
-template <bool TANGENT>
-__global__ void __launch_bounds__(TANGENT ? kJvpTM * kJvpTN : kFwdTM * kFwdTN)
+template <bool TANGENT, int TM, int TN>
+__global__ void __launch_bounds__(TM * TN)
 k_face_fwd_multi(const GridDesc* __restrict__ desc, int G) {
     const int t = (int)xcd_tile(blockIdx.x, gridDim.x);
     int lo = 0, hi = G - 1;   // last grid whose first tile is <= t
@@ -1799,11 +1791,11 @@ k_face_fwd_multi(const GridDesc* __restrict__ desc, int G) {
     const Grid g{e.d.M, e.d.N, e.d.M * e.d.N, (e.d.M + 1) * (e.d.N + 1)};
     const int tl = t - e.tile0;
     if (TANGENT)
-        face_fwd_tile<Dual<2>, double, kJvpTM, kJvpTN>(
+        face_fwd_tile<Dual<2>, double, TM, TN>(
             e.d.prm, e.fs, LdDualU<2>{e.d.X, e.d.V, e.d.U, e.d.dU, e.d.mach, g},
             StoreDualB<2>{e.d.dR, e.d.R, g.C}, tl % e.tiles_m, tl / e.tiles_m, 0);
     else
-        face_fwd_tile<double, double, kFwdTM, kFwdTN>(
+        face_fwd_tile<double, double, TM, TN>(
             e.d.prm, e.fs, LdReal{e.d.X, e.d.V, e.d.U, e.d.mach, g}, StoreRealB{e.d.R, g.C},
             tl % e.tiles_m, tl / e.tiles_m, 0);
 }
@@ -1898,17 +1890,9 @@ int launch_vjp_shape(const pea_params& P, const Ld& ld, const double* w, double*
     }
 }
 
-// Tile of the face-based adjoint (cells per workgroup of 256 threads): its TM (TN + 3) angular
-// and (TM + 3) TN radial faces should each fit one round of 256 threads.  Overridable at build
-// time like the forward tiles.
-#ifndef PEA_VJP_TILE_M
-#define PEA_VJP_TILE_M 14
-#define PEA_VJP_TILE_N 15
-#endif
-constexpr int kVjpFaceTM = PEA_VJP_TILE_M, kVjpFaceTN = PEA_VJP_TILE_N;
-
 // (dR/dU)^T w of G grids, one workgroup per tile as k_face_fwd_multi: a descriptor's dU holds
 // the seed w and dR receives the product.  Every grid needs N >= the adjoint tile's width + 3.
+template <int TM, int TN>
 __global__ void __launch_bounds__(256)
 k_face_vjp_multi(const GridDesc* __restrict__ desc, int G) {
     const int t = (int)xcd_tile(blockIdx.x, gridDim.x);
@@ -1921,19 +1905,147 @@ k_face_vjp_multi(const GridDesc* __restrict__ desc, int G) {
     const GridDesc e = desc[lo];
     const Grid g{e.d.M, e.d.N, e.d.M * e.d.N, (e.d.M + 1) * (e.d.N + 1)};
     const int tl = t - e.tile0;
-    face_vjp_tile<kVjpFaceTM, kVjpFaceTN, 256>(
+    face_vjp_tile<TM, TN, 256>(
         e.d.prm, e.fs, LdReal{e.d.X, e.d.V, e.d.U, e.d.mach, g}, e.d.dU, e.d.dR,
-        (tl % e.tiles_m) * kVjpFaceTM, (tl / e.tiles_m) * kVjpFaceTN);
+        (tl % e.tiles_m) * TM, (tl / e.tiles_m) * TN);
+}
+
+// ------------------------------------------------------------------ face-kernel tiles
+// Tile shapes of the face-based kernels among which the tuner chooses, per grid size (kind 0:
+// residual, 1: states tangent, 2: states adjoint; kinds 3-5: the same kernels in multi-grid
+// launches).  The first shape of each list is the default.  An adjoint tile TM x TN takes its
+// TM (TN + 3) angular and (TM + 3) TN radial faces in one round of 256 threads each and needs
+// N >= TN + 3.
+constexpr TileShape kFaceTiles[3][4] = {
+    {{16, 16}, {32, 8}, {32, 16}, {16, 32}},
+    {{16, 16}, {16, 8}, {32, 8}, {8, 16}},
+    {{14, 15}, {16, 13}, {18, 11}, {20, 9}},
+};
+constexpr int kNumFaceTiles = 4;
+struct FaceChoice {
+    int kind, M, N, tile;
+};
+std::mutex g_face_mutex;
+FaceChoice g_face_tiles[256];
+int g_nface_tiles = 0;
+int g_face_global[6] = {-1, -1, -1, -1, -1, -1};
+
+bool face_tile_fits(int kind, int tile, int N) {
+    return tile >= 0 && tile < kNumFaceTiles && (kind % 3 != 2 || N >= kFaceTiles[2][tile].tn + 3);
+}
+
+// The registered tile of `kind` for an M x N grid (the process-wide choice first), else the
+// first shape of the list that fits; -1 when no adjoint tile fits the grid.
+int face_tile(int kind, int M, int N) {
+    {
+        std::lock_guard<std::mutex> lk(g_face_mutex);
+        if (face_tile_fits(kind, g_face_global[kind], N)) return g_face_global[kind];
+        for (int i = 0; i < g_nface_tiles; ++i)
+            if (g_face_tiles[i].kind == kind && g_face_tiles[i].M == M && g_face_tiles[i].N == N &&
+                face_tile_fits(kind, g_face_tiles[i].tile, N))
+                return g_face_tiles[i].tile;
+    }
+    for (int t = 0; t < kNumFaceTiles; ++t)
+        if (face_tile_fits(kind, t, N)) return t;
+    return -1;
+}
+
+template <int TM, int TN>
+void launch_fwd_real_t(const pea_params& P, const LdReal& ld, double* R, int nstates,
+                       hipStream_t st) {
+    k_face_fwd<double, double, TM, TN><<<fwd_grid(ld.g, nstates, TM, TN), TM * TN, 0, st>>>(
+        P, fs_dir(P), ld, StoreRealB{R, ld.g.C});
+}
+// residual of nstates states, tile from the registry
+void launch_fwd_real(const pea_params& P, const LdReal& ld, double* R, int nstates,
+                     hipStream_t st) {
+    switch (face_tile(0, ld.g.M, ld.g.N)) {
+    case 1: launch_fwd_real_t<32, 8>(P, ld, R, nstates, st); break;
+    case 2: launch_fwd_real_t<32, 16>(P, ld, R, nstates, st); break;
+    case 3: launch_fwd_real_t<16, 32>(P, ld, R, nstates, st); break;
+    default: launch_fwd_real_t<16, 16>(P, ld, R, nstates, st); break;
+    }
+}
+
+template <int TM, int TN>
+void launch_jvp_u_t(const pea_params& P, const LdDualU<2>& ld, const StoreDualB<2>& out, int k,
+                    hipStream_t st) {
+    k_face_fwd<Dual<2>, double, TM, TN><<<fwd_grid(ld.g, k, TM, TN), TM * TN, 0, st>>>(
+        P, fs_dir(P), ld, out);
+}
+// states tangent of k complex directions (one per grid z), tile from the registry
+void launch_jvp_u(const pea_params& P, const LdDualU<2>& ld, const StoreDualB<2>& out, int k,
+                  hipStream_t st) {
+    switch (face_tile(1, ld.g.M, ld.g.N)) {
+    case 1: launch_jvp_u_t<16, 8>(P, ld, out, k, st); break;
+    case 2: launch_jvp_u_t<32, 8>(P, ld, out, k, st); break;
+    case 3: launch_jvp_u_t<8, 16>(P, ld, out, k, st); break;
+    default: launch_jvp_u_t<16, 16>(P, ld, out, k, st); break;
+    }
+}
+
+template <int TM, int TN>
+void launch_vjp_u_t(const pea_params& P, const LdReal& ld, const double* w, double* gU, int k,
+                    hipStream_t st) {
+    const dim3 grid((ld.g.M + TM - 1) / TM, (ld.g.N + TN - 1) / TN, k);
+    k_face_vjp<TM, TN, 256><<<grid, 256, 0, st>>>(P, fs_dir(P), ld, w, gU);
+}
+// states adjoint of k seeds (one per grid z); false when no adjoint tile fits the grid
+bool launch_vjp_u(const pea_params& P, const LdReal& ld, const double* w, double* gU, int k,
+                  hipStream_t st) {
+    switch (face_tile(2, ld.g.M, ld.g.N)) {
+    case 0: launch_vjp_u_t<14, 15>(P, ld, w, gU, k, st); return true;
+    case 1: launch_vjp_u_t<16, 13>(P, ld, w, gU, k, st); return true;
+    case 2: launch_vjp_u_t<18, 11>(P, ld, w, gU, k, st); return true;
+    case 3: launch_vjp_u_t<20, 9>(P, ld, w, gU, k, st); return true;
+    default: return false;
+    }
+}
+
+// The tile a multi-grid workspace was prepared with (pea_multi_launch must use the same).
+std::mutex g_multi_mutex;
+struct MultiTile {
+    const void* ws;
+    int mode, tile;
+};
+std::vector<MultiTile> g_multi_tiles;
+
+template <bool TANGENT, int TM, int TN>
+void launch_fwd_multi_t(const GridDesc* d, int G, int total, hipStream_t st) {
+    k_face_fwd_multi<TANGENT, TM, TN><<<(unsigned)total, TM * TN, 0, st>>>(d, G);
+}
+template <bool TANGENT>
+void launch_fwd_multi(int tile, const GridDesc* d, int G, int total, hipStream_t st) {
+    if (TANGENT) {
+        switch (tile) {
+        case 1: launch_fwd_multi_t<TANGENT, 16, 8>(d, G, total, st); break;
+        case 2: launch_fwd_multi_t<TANGENT, 32, 8>(d, G, total, st); break;
+        case 3: launch_fwd_multi_t<TANGENT, 8, 16>(d, G, total, st); break;
+        default: launch_fwd_multi_t<TANGENT, 16, 16>(d, G, total, st); break;
+        }
+    } else {
+        switch (tile) {
+        case 1: launch_fwd_multi_t<TANGENT, 32, 8>(d, G, total, st); break;
+        case 2: launch_fwd_multi_t<TANGENT, 32, 16>(d, G, total, st); break;
+        case 3: launch_fwd_multi_t<TANGENT, 16, 32>(d, G, total, st); break;
+        default: launch_fwd_multi_t<TANGENT, 16, 16>(d, G, total, st); break;
+        }
+    }
+}
+void launch_vjp_multi(int tile, const GridDesc* d, int G, int total, hipStream_t st) {
+    switch (tile) {
+    case 1: k_face_vjp_multi<16, 13><<<(unsigned)total, 256, 0, st>>>(d, G); break;
+    case 2: k_face_vjp_multi<18, 11><<<(unsigned)total, 256, 0, st>>>(d, G); break;
+    case 3: k_face_vjp_multi<20, 9><<<(unsigned)total, 256, 0, st>>>(d, G); break;
+    default: k_face_vjp_multi<14, 15><<<(unsigned)total, 256, 0, st>>>(d, G); break;
+    }
 }
 
 template <class T, class Ld>
 int launch_vjp(const pea_params& P, const Ld& ld, const double* w, double* gU, double* gX,
                double* gV, double* gmach, void* ws, hipStream_t st) {
     if constexpr (std::is_same<T, double>::value && std::is_same<Ld, LdReal>::value) {
-        if (gU && !gX && !gV && !gmach && ld.g.N >= kVjpFaceTN + 3 && g_face_vjp) {
-            const dim3 grid((ld.g.M + kVjpFaceTM - 1) / kVjpFaceTM,
-                            (ld.g.N + kVjpFaceTN - 1) / kVjpFaceTN);
-            k_face_vjp<kVjpFaceTM, kVjpFaceTN, 256><<<grid, 256, 0, st>>>(P, fs_dir(P), ld, w, gU);
+        if (gU && !gX && !gV && !gmach && g_face_vjp && launch_vjp_u(P, ld, w, gU, 1, st)) {
             PEA_LAUNCH_CHECK("k_face_vjp");
             return PEA_OK;
         }
@@ -2030,6 +2142,41 @@ int pea_set_tile(int M, int N, int tile) {
     return PEA_OK;
 }
 
+int pea_num_face_tiles(void) { return kNumFaceTiles; }
+
+int pea_set_face_tile(int kind, int M, int N, int tile) {
+    if (kind < 0 || kind > 5) return fail(PEA_ERR_ARG, "kind %d out of range", kind);
+    if (tile < -1 || tile >= kNumFaceTiles) return fail(PEA_ERR_ARG, "tile %d out of range", tile);
+    std::lock_guard<std::mutex> lk(g_face_mutex);
+    if (M == 0 && N == 0) {
+        g_face_global[kind] = tile;
+        return PEA_OK;
+    }
+    if (kind >= 3) return fail(PEA_ERR_ARG, "multi-grid tiles are process-wide (M = N = 0)");
+    if (int rc = check_dims(M, N)) return rc;
+    if (tile >= 0 && !face_tile_fits(kind, tile, N))
+        return fail(PEA_ERR_ARG, "tile %d does not fit N=%d", tile, N);
+    for (int i = 0; i < g_nface_tiles; ++i)
+        if (g_face_tiles[i].kind == kind && g_face_tiles[i].M == M && g_face_tiles[i].N == N) {
+            g_face_tiles[i].tile = tile;
+            return PEA_OK;
+        }
+    if (tile < 0) return PEA_OK;
+    if (g_nface_tiles == 256) return fail(PEA_ERR_ARG, "tile registry full");
+    g_face_tiles[g_nface_tiles++] = FaceChoice{kind, M, N, tile};
+    return PEA_OK;
+}
+
+int pea_get_face_tile(int kind, int M, int N, int* tm, int* tn) {
+    if (kind < 0 || kind > 5) return fail(PEA_ERR_ARG, "kind %d out of range", kind);
+    const int t = kind >= 3 ? face_tile(kind, 0, 1 << 30) : face_tile(kind, M, N);
+    if (t >= 0 && tm && tn) {
+        *tm = kFaceTiles[kind % 3][t].tm;
+        *tn = kFaceTiles[kind % 3][t].tn;
+    }
+    return t;
+}
+
 int pea_get_tile(int M, int N) {
     if (check_dims(M, N)) return -1;
     return pick_tile(make_grid(M, N), false);
@@ -2050,9 +2197,7 @@ int pea_compute_odes(const pea_params* prm, double mach, int M, int N, const dou
     if (int rc = check_dims(M, N)) return rc;
     if (!prm || !X || !V || !U || !R) return fail(PEA_ERR_NULL, "null argument");
     const Grid g = make_grid(M, N);
-    k_face_fwd<double, double, kFwdTM, kFwdTN><<<fwd_grid(g, 1), kFwdTM * kFwdTN, 0,
-                                                 as_stream(stream)>>>(
-        *prm, fs_dir(*prm), LdReal{X, V, U, mach, g}, StoreRealB{R, g.C});
+    launch_fwd_real(*prm, LdReal{X, V, U, mach, g}, R, 1, as_stream(stream));
     PEA_LAUNCH_CHECK("k_face_fwd");
     return PEA_OK;
 }
@@ -2064,9 +2209,7 @@ int pea_compute_odes_batch(const pea_params* prm, double mach, int M, int N, int
     if (!prm || !X || !V || !U || !R) return fail(PEA_ERR_NULL, "null argument");
     if (nstates < 1 || nstates > 65535) return fail(PEA_ERR_ARG, "nstates %d", nstates);
     const Grid g = make_grid(M, N);
-    k_face_fwd<double, double, kFwdTM, kFwdTN><<<fwd_grid(g, nstates), kFwdTM * kFwdTN, 0,
-                                                 as_stream(stream)>>>(
-        *prm, fs_dir(*prm), LdReal{X, V, U, mach, g}, StoreRealB{R, g.C});
+    launch_fwd_real(*prm, LdReal{X, V, U, mach, g}, R, nstates, as_stream(stream));
     PEA_LAUNCH_CHECK("k_face_fwd");
     return PEA_OK;
 }
@@ -2078,8 +2221,9 @@ int pea_multi_prepare(int G, const pea_grid* grids, int tangent, void* workspace
     if (G < 1) return fail(PEA_ERR_DIMS, "no grids");
     if (!grids || !workspace || !total_tiles) return fail(PEA_ERR_NULL, "null argument");
     if (tangent < 0 || tangent > 2) return fail(PEA_ERR_ARG, "mode %d", tangent);
-    const int tm = tangent == 2 ? kVjpFaceTM : tangent ? kJvpTM : kFwdTM;
-    const int tn = tangent == 2 ? kVjpFaceTN : tangent ? kJvpTN : kFwdTN;
+    const int tile = face_tile(3 + tangent, 0, 1 << 30);
+    const TileShape ts = kFaceTiles[tangent][tile < 0 ? 0 : tile];
+    const int tm = ts.tm, tn = ts.tn;
     std::vector<GridDesc> d(G);
     long long total = 0;
     for (int i = 0; i < G; ++i) {
@@ -2087,9 +2231,9 @@ int pea_multi_prepare(int G, const pea_grid* grids, int tangent, void* workspace
         if (int rc = check_dims(q.M, q.N)) return rc;
         if (!q.X || !q.V || !q.U || (tangent ? (!q.dU || !q.dR) : !q.R))
             return fail(PEA_ERR_NULL, "null array in a grid descriptor");
-        if (tangent == 2 && q.N < kVjpFaceTN + 3)
+        if (tangent == 2 && q.N < tn + 3)
             return fail(PEA_ERR_DIMS, "multi-grid adjoint needs N >= %d (grid %d: N=%d)",
-                        kVjpFaceTN + 3, i, q.N);
+                        tn + 3, i, q.N);
         d[i].d = q;
         d[i].fs = fs_dir(q.prm);
         d[i].tiles_m = (q.M + tm - 1) / tm;
@@ -2104,6 +2248,14 @@ int pea_multi_prepare(int G, const pea_grid* grids, int tangent, void* workspace
         hipStreamSynchronize(st) != hipSuccess)
         return fail(PEA_ERR_HIP, "descriptor upload");
     *total_tiles = (int)total;
+    std::lock_guard<std::mutex> lk(g_multi_mutex);
+    for (auto& e : g_multi_tiles)
+        if (e.ws == workspace) {
+            e.mode = tangent;
+            e.tile = tile < 0 ? 0 : tile;
+            return PEA_OK;
+        }
+    g_multi_tiles.push_back(MultiTile{workspace, tangent, tile < 0 ? 0 : tile});
     return PEA_OK;
 }
 
@@ -2112,16 +2264,22 @@ int pea_multi_launch(int G, int total_tiles, int tangent, const void* workspace,
     if (!workspace) return fail(PEA_ERR_NULL, "null argument");
     if (tangent < 0 || tangent > 2) return fail(PEA_ERR_ARG, "mode %d", tangent);
     const GridDesc* d = static_cast<const GridDesc*>(workspace);
+    int tile = -1;
+    {
+        std::lock_guard<std::mutex> lk(g_multi_mutex);
+        for (const auto& e : g_multi_tiles)
+            if (e.ws == workspace && e.mode == tangent) tile = e.tile;
+    }
+    if (tile < 0) return fail(PEA_ERR_ARG, "workspace not prepared for mode %d", tangent);
     if (tangent == 2) {
-        k_face_vjp_multi<<<(unsigned)total_tiles, 256, 0, as_stream(stream)>>>(d, G);
+        launch_vjp_multi(tile, d, G, total_tiles, as_stream(stream));
         PEA_LAUNCH_CHECK("k_face_vjp_multi");
         return PEA_OK;
     }
-    const int nt = tangent ? kJvpTM * kJvpTN : kFwdTM * kFwdTN;
     if (tangent)
-        k_face_fwd_multi<true><<<(unsigned)total_tiles, nt, 0, as_stream(stream)>>>(d, G);
+        launch_fwd_multi<true>(tile, d, G, total_tiles, as_stream(stream));
     else
-        k_face_fwd_multi<false><<<(unsigned)total_tiles, nt, 0, as_stream(stream)>>>(d, G);
+        launch_fwd_multi<false>(tile, d, G, total_tiles, as_stream(stream));
     PEA_LAUNCH_CHECK("k_face_fwd_multi");
     return PEA_OK;
 }
@@ -2168,9 +2326,8 @@ int pea_jvp_odes(const pea_params* prm, double mach, int M, int N, const double*
     const int bs = cell_block(g.C);
     (void)bs;
     if (dU && !dX && !dV && !dmach)
-        k_face_fwd<Dual<2>, double, kJvpTM, kJvpTN><<<fwd_grid(g, 1, kJvpTM, kJvpTN),
-                                                      kJvpTM * kJvpTN, 0, as_stream(stream)>>>(
-            *prm, fs_dir(*prm), LdDualU<2>{X, V, U, dU, mach, g}, StoreDualB<2>{dR, R, g.C});
+        launch_jvp_u(*prm, LdDualU<2>{X, V, U, dU, mach, g}, StoreDualB<2>{dR, R, g.C}, 1,
+                     as_stream(stream));
     else
         k_face_fwd<Dual<2>, Dual<2>, kJvpTM, kJvpTN><<<fwd_grid(g, 1, kJvpTM, kJvpTN),
                                                        kJvpTM * kJvpTN, 0, as_stream(stream)>>>(
@@ -2193,9 +2350,8 @@ int pea_jvp_odes_multi(const pea_params* prm, double mach, int M, int N, int k,
     const bool u_only = dU && !dX && !dV && !dmach;
     if (u_only) {   // one face-kernel launch, one complex direction per grid z
         if (k < 1 || k > 65535) return fail(PEA_ERR_ARG, "k must be 1..65535 (got %d)", k);
-        k_face_fwd<Dual<2>, double, kJvpTM, kJvpTN><<<fwd_grid(g, k, kJvpTM, kJvpTN),
-                                                      kJvpTM * kJvpTN, 0, st>>>(
-            *prm, fs_dir(*prm), LdDualU<2>{X, V, U, dU, mach, g}, StoreDualB<2>{dR, nullptr, g.C});
+        launch_jvp_u(*prm, LdDualU<2>{X, V, U, dU, mach, g}, StoreDualB<2>{dR, nullptr, g.C}, k,
+                     st);
         PEA_LAUNCH_CHECK("k_face_fwd");
         return PEA_OK;
     }
@@ -2314,9 +2470,7 @@ int pea_vjp_odes_states_multi(const pea_params* prm, double mach, int M, int N, 
     const Grid g = make_grid(M, N);
     const LdReal ld{X, V, U, mach, g};
     hipStream_t st = as_stream(stream);
-    if (N >= kVjpFaceTN + 3 && g_face_vjp) {   // one launch, one seed per grid z
-        const dim3 grid((M + kVjpFaceTM - 1) / kVjpFaceTM, (N + kVjpFaceTN - 1) / kVjpFaceTN, k);
-        k_face_vjp<kVjpFaceTM, kVjpFaceTN, 256><<<grid, 256, 0, st>>>(*prm, fs_dir(*prm), ld, w, gU);
+    if (g_face_vjp && launch_vjp_u(*prm, ld, w, gU, k, st)) {   // one launch, seed per grid z
         PEA_LAUNCH_CHECK("k_face_vjp");
         return PEA_OK;
     }

@@ -334,6 +334,109 @@ def autotune_vjp(p: Params, mach: float, g: Grid, X, V, U, reps: int = 5) -> int
     return best
 
 
+# ------------------------------------------------------------------------ face-kernel autotuner
+FACE_KINDS = {"residual": 0, "jvp": 1, "vjp": 2}
+_FACE_TUNED: dict = {}       # (kind, M, N) -> tile, also (kind + 3, 0, 0) for multi-grid launches
+
+
+def face_tile(kind: int, M: int = 0, N: int = 0) -> tuple[int, int, int]:
+    """(tile index, TM, TN) the face kernel of ``kind`` uses for an M x N grid."""
+    tm, tn = ctypes.c_int(0), ctypes.c_int(0)
+    t = _lib.load().pea_get_face_tile(kind, M, N, ctypes.byref(tm), ctypes.byref(tn))
+    return t, tm.value, tn.value
+
+
+def set_face_tile(kind: int, M: int, N: int, tile: int) -> None:
+    _lib.call("pea_set_face_tile", kind, M, N, tile)
+    if tile < 0:
+        _FACE_TUNED.pop((kind, M, N), None)
+    else:
+        _FACE_TUNED[(kind, M, N)] = tile
+
+
+def _time_events(fn, reps: int) -> float:
+    fn()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    b.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def autotune_face(p: Params, mach: float, g: Grid, X, V, U, kinds=("residual", "jvp", "vjp"),
+                  reps: int = 10, batch: int = 1) -> dict:
+    """Times every tile shape of the face kernels (residual, states tangent, states adjoint) on
+    this grid, ``batch`` states / directions / seeds per launch, and registers the fastest for
+    grids of this size (``pea_set_face_tile``).  The choices are cached per (kind, M, N) for the
+    process; :func:`save_tuning` / :func:`load_tuning` keep them across processes.  Returns
+    {kind: (tile, {tile: ms})}."""
+    lib = _lib.load()
+    gen = torch.Generator(device=U.device).manual_seed(0)
+    Ub = U.expand(batch, *U.shape).contiguous()
+    Rb = torch.empty_like(Ub)
+    d = torch.randn((batch, *g.cell_shape), dtype=C128, device=U.device, generator=gen)
+    out = torch.empty_like(d)
+    runs = {
+        "residual": lambda: compute_odes_batch(p, mach, g, X, V, Ub, Rb),
+        "jvp": lambda: jvp_odes_multi(p, mach, g, X, V, U, dU=d, dR=out),
+        "vjp": lambda: vjp_odes_states_multi(p, mach, g, X, V, U, d, gU=out),
+    }
+    result = {}
+    for name in kinds:
+        kind = FACE_KINDS[name]
+        times = {}
+        for t in range(lib.pea_num_face_tiles()):
+            if lib.pea_set_face_tile(kind, g.M, g.N, t) != 0:
+                continue
+            times[t] = _time_events(runs[name], reps)
+        best = min(times, key=times.get)
+        set_face_tile(kind, g.M, g.N, best)
+        result[name] = (best, times)
+    return result
+
+
+def autotune_multigrid(items, reps: int = 5) -> dict:
+    """Times every tile shape of the multi-grid residual, states tangent and states adjoint
+    launches over ``items`` (params, mach, X, V, U, dU) and registers the fastest process-wide
+    (pea_set_face_tile kinds 3-5).  Returns {mode: (tile, {tile: ms})}."""
+    lib = _lib.load()
+    base = [it[:5] for it in items]
+    result = {}
+    for mode, name in ((0, "residual"), (1, "jvp"), (2, "vjp")):
+        times = {}
+        for t in range(lib.pea_num_face_tiles()):
+            _lib.call("pea_set_face_tile", 3 + mode, 0, 0, t)
+            try:
+                mg = MultiGrid(items if mode else base, tangent=mode == 1, adjoint=mode == 2)
+            except _lib.PeaError:          # an adjoint tile too wide for one of the grids
+                continue
+            times[t] = _time_events(mg.run, reps)
+        best = min(times, key=times.get)
+        set_face_tile(3 + mode, 0, 0, best)
+        result[name] = (best, times)
+    return result
+
+
+def save_tuning(path) -> None:
+    """Writes the registered face-tile choices as JSON."""
+    import json
+    rows = [{"kind": k, "M": M, "N": N, "tile": t} for (k, M, N), t in sorted(_FACE_TUNED.items())]
+    with open(path, "w") as fh:
+        json.dump({"face_tiles": rows}, fh, indent=1)
+
+
+def load_tuning(path) -> int:
+    """Registers the choices of a JSON file written by :func:`save_tuning`; returns how many."""
+    import json
+    with open(path) as fh:
+        rows = json.load(fh)["face_tiles"]
+    for r in rows:
+        set_face_tile(int(r["kind"]), int(r["M"]), int(r["N"]), int(r["tile"]))
+    return len(rows)
+
+
 def vjp_odes(p: Params, mach: float, g: Grid, X, V, U, w, want=("states",), out=None):
     """Transposed products with complex seed w (no conjugation).  Returns a dict with the
     requested subset of 'states', 'vertices', 'velocities', 'mach'."""

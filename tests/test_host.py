@@ -206,3 +206,31 @@ def test_f2py_compatible_module_has_the_reference_procedures():
     disc.aoa[...] = 1.5                    # as core.py:258-263 writes them
     assert disc._params().aoa_deg == 1.5
     disc.aoa[...] = 0.0
+
+
+def test_face_tile_registry_and_json_cache(tmp_path):
+    """The face-kernel tile registry (host side only: no launch) and the tuner's JSON cache."""
+    from pyeulerale_amd import _lib
+    from pyeulerale_amd import kernels as K
+    lib = _lib.load()
+    n = lib.pea_num_face_tiles()
+    assert n >= 2
+    assert K.face_tile(0, 512, 1024) == (0, 16, 16)           # defaults
+    assert K.face_tile(2, 512, 1024) == (0, 14, 15)
+    assert K.face_tile(2, 8, 10)[0] == -1                      # no adjoint tile fits N = 10
+    assert K.face_tile(2, 8, 14)[0] == 2                       # the 18 x 11 tile needs N >= 14
+    with pytest.raises(_lib.PeaError):
+        K.set_face_tile(2, 8, 14, 0)                           # 14 x 15 needs N >= 18
+    with pytest.raises(_lib.PeaError):
+        K.set_face_tile(1, 64, 128, n)
+    K.set_face_tile(1, 64, 128, 2)
+    K.set_face_tile(4, 0, 0, 1)
+    path = tmp_path / "tiles.json"
+    K.save_tuning(path)
+    K.set_face_tile(1, 64, 128, -1)
+    K.set_face_tile(4, 0, 0, -1)
+    assert K.face_tile(1, 64, 128)[0] == 0 and K.face_tile(4)[0] == 0
+    assert K.load_tuning(path) == 2
+    assert K.face_tile(1, 64, 128) == (2, 32, 8) and K.face_tile(4)[0] == 1
+    K.set_face_tile(1, 64, 128, -1)
+    K.set_face_tile(4, 0, 0, -1)
