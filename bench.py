@@ -196,6 +196,11 @@ def products(torch, K, A, Rv, thickness, mach, aoa, alpha, alpha_dot, far, reps)
         p, mach, g, Xd, Vd, Ud, db, gU=ob), max(2, reps // 5)) / kb
     t["cs_dir_per_product_batch32"] = graph_time_ms(torch, lambda: K.cs_jvp_odes_multi(
         p, mach, g, Xd, Vd, Ud, db, dR=ob), max(2, reps // 5)) / kb
+    # the same 32 directions with each tile's primal part evaluated once for all of them
+    t["jvp_dirs_per_product_batch32"] = graph_time_ms(torch, lambda: K.jvp_odes_dirs(
+        p, mach, g, Xd, Vd, Ud, db, dR=ob), max(2, reps // 5)) / kb
+    err_dirs = rel(K.jvp_odes_dirs(p, mach, g, Xd, Vd, Ud, db),
+                   K.jvp_odes_multi(p, mach, g, Xd, Vd, Ud, dU=db))
     del db, ob
     # accuracy of the hand-written products against complex step (h = 1e-12)
     jvp = K.jvp_odes(p, mach, g, Xd, Vd, Ud, dU=dU)
@@ -216,6 +221,13 @@ def products(torch, K, A, Rv, thickness, mach, aoa, alpha, alpha_dot, far, reps)
         "speedup_vjp_vs_cs_asm_rev": t["cs_asm_apply_rev"] / t["vjp"],
         "speedup_vjp_batch32_vs_cs_dir_batch32": t["cs_dir_per_product_batch32"] / t["vjp_per_product_batch32"],
         "speedup_jvp_batch32_vs_cs_dir_batch32": t["cs_dir_per_product_batch32"] / t["jvp_per_product_batch32"],
+        "speedup_jvp_dirs_batch32_vs_cs_dir_batch32":
+            t["cs_dir_per_product_batch32"] / t["jvp_dirs_per_product_batch32"],
+        "rel_err_jvp_dirs_vs_dual_batch32": err_dirs,
+        # bytes per product with 32 directions per launch: the direction read and the product
+        # written (2 x 64 B per cell), the state and geometry (32 B per cell and vertex) once
+        "jvp_dirs_hbm_frac": (128 * C + (32 * C + 32 * P) / kb) /
+                             (t["jvp_dirs_per_product_batch32"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "speedup_vjp_vs_cs_linearize_plus_rev": (t["cs_linearize"] + t["cs_asm_apply_rev"]) / t["vjp"],
         "jvp_hbm_frac": (160 * C + 32 * P) / (t["jvp"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "vjp_hbm_frac": (160 * C + 32 * P) / (t["vjp"] * 1e-3) / 1e9 / HBM_PEAK_GBS,

@@ -47,6 +47,7 @@ _SIGS = {
     "pea_check_states": [_I, _I, _P, _P, _P, _P],
     "pea_jvp_odes": [_P, _D, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "pea_jvp_odes_multi": [_P, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "pea_jvp_odes_states_dirs": [_P, _D, _I, _I, _I, _P, _P, _P, _P, _P, _I, _P],
     "pea_vjp_odes_states_multi": [_P, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P],
     "pea_jvp_forces": [_I, _I, _P, _P, _P, _P, _P, _P],
     "pea_vjp_odes": [_P, _D, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],

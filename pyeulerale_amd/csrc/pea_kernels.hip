@@ -17,6 +17,7 @@
 
 #include "pea_adjoint.hpp"
 #include "pea_face.hpp"
+#include "pea_linear.hpp"
 
 using namespace pea;
 
@@ -83,6 +84,28 @@ constexpr int kXcdCount = 8;
 __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned total) {
     const unsigned q = total / kXcdCount, r = total % kXcdCount, x = b % kXcdCount;
     return x * q + (x < r ? x : r) + b / kXcdCount;
+}
+
+// Tile (bx, by) and batch item bz (state, direction or seed of one grid) of this workgroup in a
+// launch over nx x ny tiles and nz items.  Items come in chunks of up to kItemChunk; a chunk's
+// tiles are numbered spatial tile after spatial tile, its items fastest, and xcd_tile() hands
+// each XCD a contiguous range of that numbering.  The items of a tile share its geometry (and,
+// for the products, its primal states), so one read of them into an XCD's L2 serves the whole
+// chunk; the tiles next to it in m and n (which share its halo) follow within kItemChunk and
+// kItemChunk * nx workgroups.  With the items slowest each XCD would take whole items, and
+// every XCD would read all of the geometry: 1.8x the algorithmic reads for 8 states at
+// 1025x513, against 1.2x this way.
+constexpr unsigned kItemChunk = 8;
+struct TileId {
+    int bx, by, bz;
+};
+__device__ __forceinline__ TileId batch_tile(unsigned nx, unsigned ny, unsigned nz) {
+    const unsigned S = nx * ny;
+    const unsigned t = xcd_tile(blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z), S * nz);
+    const unsigned c = t / (S * kItemChunk), r = t - c * S * kItemChunk;
+    const unsigned cz = min(kItemChunk, nz - c * kItemChunk);   // items in this chunk
+    const unsigned s = r / cz;
+    return {(int)(s % nx), (int)(s / nx), (int)(c * kItemChunk + r - s * cz)};
 }
 
 // cell index (n*M + m) of stencil slot `slot` of cell (m, n); radial neighbours are only
@@ -732,11 +755,366 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const FsDir& 
 template <class T, class G, int TM, int TN, class Ld, class Store>
 __global__ void __launch_bounds__(TM * TN)
 k_face_fwd(pea_params P, FsDir fs, Ld ld, Store store) {
-    const unsigned nx = gridDim.x, ny = gridDim.y;
-    const unsigned t = xcd_tile(blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z),
-                                nx * ny * gridDim.z);
-    face_fwd_tile<T, G, TM, TN>(P, fs, ld, store, (int)(t % nx), (int)(t / nx % ny),
-                                (int)(t / (nx * ny)));
+    const TileId t = batch_tile(gridDim.x, gridDim.y, gridDim.z);
+    face_fwd_tile<T, G, TM, TN>(P, fs, ld, store, t.bx, t.by, t.bz);
+}
+
+// ------------------------------------------------------------------ many directions at one state
+// States tangent of many real directions at one state (pea_jvp_odes_states_dirs).  One workgroup
+// per tile, as face_fwd_tile, but the primal part is done once per tile: the states, pressures and
+// 1/rho of the grown tile go to LDS, and every face the tile evaluates gets its FaceLin record
+// (pea_linear.hpp): in registers for the f and o faces a thread owns, in LDS for the tile-edge
+// faces of the first wave.  Then the workgroup walks real directions j0 .. j1-1 (the real and
+// imaginary parts of complex directions j/2): it stages the direction's states of the grown tile
+// and their pressure tangents in LDS, applies each face's record, hands the other side's flux to
+// the neighbouring cell as face_fwd_tile does, and stores -(sum of the four face fluxes)/area.
+// Every global access is a buffer access bounded by its array.
+__device__ __forceinline__ void lin_put(double* base, int stride, int i, const FaceLin& f) {
+    const double w[kLinWords] = {f.n0,   f.n1,   f.hvn,  f.L.lam, f.L.s, f.L.ib, f.L.aC,
+                                 f.L.bC, f.R.lam, f.R.s, f.R.ib, f.R.aC, f.R.bC,
+                                 __builtin_bit_cast(double, (unsigned long long)f.bits)};
+#pragma unroll
+    for (int k = 0; k < kLinWords; ++k) base[k * stride + i] = w[k];
+}
+__device__ __forceinline__ FaceLin lin_get(const double* base, int stride, int i) {
+    double w[kLinWords];
+#pragma unroll
+    for (int k = 0; k < kLinWords; ++k) w[k] = base[k * stride + i];
+    FaceLin f;
+    f.n0 = w[0], f.n1 = w[1], f.hvn = w[2];
+    f.L = SideLin{w[3], w[4], w[5], w[6], w[7]};
+    f.R = SideLin{w[8], w[9], w[10], w[11], w[12]};
+    f.bits = (unsigned)__builtin_bit_cast(unsigned long long, w[13]);
+    return f;
+}
+typedef unsigned int pea_u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, double x, int byte_off,
+                                          int plane_bytes) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(pea_u32x2, x), r, byte_off,
+                                          plane_bytes, 0);
+}
+
+template <int TM, int TN>
+__device__ __forceinline__ void face_dirs_tile(const pea_params& P, const FsDir& fs,
+                                               const LdReal& l, const double* dU, double* dR,
+                                               int ndir, int j0, int j1, int bx, int by) {
+    constexpr int NT = TM * TN, BM = TM + 4, BN = TN + 4, BOX = BM * BN;
+    static_assert(TM + TN <= NT, "the tile-edge faces take one thread each");
+    static_assert(64 % TM == 0 && NT % 64 == 0, "a wave holds whole tile columns");
+    constexpr int CPW = 64 / TM, SB = NT / 64 * TM, NPASS = (BOX + NT - 1) / NT;
+    constexpr int NE = TM + TN;
+    // LDS: primal states, pressures and 1/rho of the grown tile; the direction's states and
+    // pressure tangents (sdP holds the primal sound speeds while the records are built); the
+    // tile-edge records; the wall-ghost map of each box column; the flux hand-offs.
+    __shared__ double sU[4 * BOX], sP[BOX], sI[BOX];
+    __shared__ double sdU[4 * BOX], sdP[BOX];
+    __shared__ double sRec[kLinWords * NE];
+    __shared__ double sW[5 * BN];
+    __shared__ double sFb[4 * SB], sFi[4 * TN];
+    double* const sC = sdP;
+    const Grid g = l.g;
+    const int m0 = bx * TM, n0 = by * TN;
+    const int tid = threadIdx.x;
+    const int tm = tid % TM, tn = tid / TM;
+    const int m = m0 + tm, n = n0 + tn;
+    const bool live = m < g.M && n < g.N;
+
+    auto vtx = [&](int mm, int nn, double* x) { l.vertex(nn * (g.M + 1) + mm, x); };
+    auto vel = [&](int mm, int nn, double* w) { l.velocity(nn * (g.M + 1) + mm, w); };
+    auto edge_of = [](const double* xa, const double* xb, const double* va, const double* vb,
+                      double* nr, double* gv) {
+        edge_normal(xa, xb, nr);
+        gv[0] = (va[0] + vb[0]) / 2.0;
+        gv[1] = (va[1] + vb[1]) / 2.0;
+    };
+    // geometry of this cell's faces and area, and of the tile-edge face of threads below NE
+    double gx[4][2], gw[3][2], ex[2][2], ew[2][2];
+    {
+        const int mc = m < g.M ? m : g.M - 1, nc = n < g.N ? n : g.N - 1;
+        vtx(mc, nc, gx[BI]);
+        vtx(mc, nc + 1, gx[FI]);
+        vtx(mc + 1, nc + 1, gx[FO]);
+        vtx(mc + 1, nc, gx[BO]);
+        vel(mc, nc + 1, gw[0]);
+        vel(mc + 1, nc + 1, gw[1]);
+        vel(mc + 1, nc, gw[2]);
+    }
+    if (tid < NE) {
+        const int mm = m0 + tid < g.M ? m0 + tid : g.M - 1;
+        const int nn = n0 + tid - TM < g.N ? n0 + tid - TM : g.N - 1;
+        int ma, na, mb, nb;
+        if (tid < TM)
+            ma = mm, na = n0, mb = mm + 1, nb = n0;
+        else if (m0 == 0)
+            ma = 0, na = nn, mb = 0, nb = nn + 1;
+        else
+            ma = m0, na = nn + 1, mb = m0, nb = nn;
+        vtx(ma, na, ex[0]);
+        vtx(mb, nb, ex[1]);
+        vel(ma, na, ew[0]);
+        vel(mb, nb, ew[1]);
+    }
+
+    // 1. primal staging.  Each thread keeps, for the box cells it stages, the coefficients of
+    // the pressure tangent dp = (gamma-1)(dE - cu dj_x - cv dj_y + ck drho) (pressure_q's) and
+    // what the cell is: 0 interior, 1 wall ghost, 2 far-field ghost, 3 outside the stencils.
+    double cu[NPASS], cv[NPASS], ck[NPASS];
+    int kind[NPASS], cell[NPASS];
+    auto pcoef = [](const double* u, double& a, double& b, double& c) {
+        const double rb = rcp_fast(u[0]);
+        const double r = (u[1] * u[1] + u[2] * u[2]) * rb;
+        a = u[1] * rb;
+        b = u[2] * rb;
+        c = 0.5 * r * rb;
+    };
+#pragma unroll
+    for (int k = 0; k < NPASS; ++k) {
+        const int i = tid + k * NT;
+        kind[k] = 3;
+        cell[k] = 0;
+        cu[k] = cv[k] = ck[k] = 0.0;
+        if (i >= BOX) continue;
+        const int bm = i % BM, bn = i / BM;
+        const int cm = m0 - 2 + bm;
+        const int cn = wrap_any(n0 - 2 + bn, g.N);
+        cell[k] = cn * g.M + (cm < 0 ? 0 : (cm >= g.M ? g.M - 1 : cm));
+        double u[4];
+        l.state(cell[k], u);
+        double s[4] = {0.0, 0.0, 0.0, 0.0}, p = 0.0, c = 0.0;
+        if (cm >= 0 && cm < g.M) {
+            kind[k] = 0;
+            cell_pc(u, p, c);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) s[v] = u[v];
+        } else if (cm == -1) {
+            kind[k] = 1;
+            double xa[2], xb[2], va[2], vb[2], nr[2], gv[2];
+            vtx(0, cn, xa);
+            vtx(0, cn + 1, xb);
+            vel(0, cn, va);
+            vel(0, cn + 1, vb);
+            edge_of(xa, xb, va, vb, nr, gv);   // wall face bi -> fi
+            wall_ghost(u, nr, gv, s);
+            p = pressure_q(s);
+            // the ghost is linear in the state: dg_{1,2} = dj - 2 n (n.(dj - gv drho)) / |n|^2
+            sW[bn] = nr[0];
+            sW[BN + bn] = nr[1];
+            sW[2 * BN + bn] = gv[0];
+            sW[3 * BN + bn] = gv[1];
+            sW[4 * BN + bn] = rcp_fast(nr[0] * nr[0] + nr[1] * nr[1]);
+        } else if (cm == g.M) {
+            kind[k] = 2;
+            double inf[4];
+            free_stream_dir(l.machv(), fs.ca, fs.sa, inf);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) s[v] = 2.0 * inf[v] - u[v];
+            p = pressure_q(s);
+        }
+        if (kind[k] < 3) pcoef(s, cu[k], cv[k], ck[k]);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sU[v * BOX + i] = s[v];
+        sP[i] = p;
+        sC[i] = c;
+        sI[i] = kind[k] < 3 ? rcp_fast(s[0]) : 0.0;
+    }
+    __syncthreads();
+
+    auto bidx = [&](int mm, int nb) { return nb * BM + (mm - m0 + 2); };   // nb: box column
+    auto prim = [&](int bLL, int bL, int bR, int bRR) {
+        return [bLL, bL, bR, bRR](int slot, int v) -> double {
+            const int b = slot == 0 ? bLL : slot == 1 ? bL : slot == 2 ? bR : bRR;
+            return sU[v * BOX + b];
+        };
+    };
+    auto dirs = [&](int bLL, int bL, int bR, int bRR) {
+        return [bLL, bL, bR, bRR](int slot, int v) -> double {
+            const int b = slot == 0 ? bLL : slot == 1 ? bL : slot == 2 ? bR : bRR;
+            return sdU[v * BOX + b];
+        };
+    };
+    auto build = [&](int bLL, int bL, int bR, int bRR, const double* nr, const double* nr2,
+                     const double* gv, bool dL, bool dRs, FaceLin& f) {
+        face_lin_build(P, prim(bLL, bL, bR, bRR), sP[bLL], sP[bL], sP[bR], sP[bRR], sC[bL],
+                       sC[bR], sI[bL], sI[bR], nr, nr2, gv, dL, dRs, f);
+    };
+
+    // 2. records.  Stencil box indices of this thread's f face (angular n | n+1) and o face
+    // (radial m | m+1, or the far-field face of m = M-1: central against the ghost row M).
+    const int bn = tn + 2;
+    const int bc = bidx(m, bn);
+    const int fLL = bc - BM, fR = bc + BM, fRR = bc + 2 * BM;
+    const bool far = m == g.M - 1;
+    const int oLL = far ? bc : bc - 1, oR = bc + 1, oRR = far ? bc + 1 : bc + 2;
+    FaceLin rf, ro;
+    double ia = 0.0;
+    if (live) {
+        double nr[2], nr2[2], gv[2];
+        edge_of(gx[FI], gx[FO], gw[0], gw[1], nr, gv);   // fi -> fo
+        edge_normal(gx[FO], gx[FI], nr2);                // R's bo -> bi
+        build(fLL, bc, fR, fRR, nr, nr2, gv, true, true, rf);
+        edge_of(gx[FO], gx[BO], gw[1], gw[2], nr, gv);   // fo -> bo
+        edge_normal(gx[BO], gx[FO], nr2);                // R's bi -> fi
+        build(oLL, bc, oR, oRR, nr, nr2, gv, !far && m != g.M - 2, !far && m + 1 != 1, ro);
+        const double d1x = gx[FO][0] - gx[BI][0], d1z = gx[FO][1] - gx[BI][1];
+        const double d2x = gx[BO][0] - gx[FI][0], d2z = gx[BO][1] - gx[FI][1];
+        ia = recip(sabs(d1x * d2z - d1z * d2x) / 2.0);
+    } else {
+        const double z[2] = {0.0, 0.0};
+        build(bc, bc, bc, bc, z, z, z, false, false, rf);
+        ro = rf;
+    }
+    // tile-edge faces: threads below TM take the angular face n0-1 | n0 of row m0 + tid,
+    // threads TM .. NE-1 the radial face m0-1 | m0 of column n0 + tid - TM (the wall face for
+    // m0 = 0: central against the wall ghost, its flux is the cell's own)
+    int eLL = 0, eL = 0, eR = 0, eRR = 0;
+    bool e_live = false, e_own = false;
+    if (tid < TM) {
+        const int mm = m0 + tid;
+        e_live = mm < g.M;
+        eLL = bidx(mm, 0), eL = bidx(mm, 1), eR = bidx(mm, 2), eRR = bidx(mm, 3);
+        if (e_live) {
+            FaceLin fe;
+            double nr[2], nr2[2], gv[2];
+            edge_of(ex[0], ex[1], ew[0], ew[1], nr, gv);   // bi -> bo
+            edge_normal(ex[1], ex[0], nr2);                // the cell's bo -> bi
+            build(eLL, eL, eR, eRR, nr, nr2, gv, false, true, fe);
+            lin_put(sRec, NE, tid, fe);
+        }
+    } else if (tid < NE) {
+        const int j = tid - TM, mm = m0, bnn = j + 2;
+        e_live = n0 + j < g.N;
+        if (m0 == 0) {
+            e_own = true;
+            eLL = eL = bidx(0, bnn);
+            eR = eRR = bidx(-1, bnn);
+        } else {
+            eLL = bidx(mm - 2, bnn), eL = bidx(mm - 1, bnn), eR = bidx(mm, bnn),
+            eRR = bidx(mm + 1, bnn);
+        }
+        if (e_live) {
+            FaceLin fe;
+            double nr[2], nr2[2], gv[2];
+            edge_of(ex[0], ex[1], ew[0], ew[1], nr, gv);   // bi -> fi (wall) or fo -> bo of L
+            edge_normal(ex[1], ex[0], nr2);
+            build(eLL, eL, eR, eRR, nr, nr2, gv, false, !e_own && mm != 1, fe);
+            lin_put(sRec, NE, tid, fe);
+        }
+    }
+    __syncthreads();
+
+    // 3. directions.  Loads and stores address one real part of the interleaved complex arrays:
+    // element (d, v, cell) part j & 1 at byte 16 ((4 d + v) C + cell) + 8 (j & 1).
+    const auto rU = buf_rsrc(dU, 64L * g.C * ndir);
+    const auto rR = buf_rsrc(dR, 64L * g.C * ndir);
+    const int cme = n * g.M + m;
+    // the direction's states of this thread's box cells, loaded one direction ahead so that
+    // the loads overlap the previous direction's face evaluations
+    double nxt[NPASS][4];
+    auto fetch = [&](int jj) {
+#pragma unroll
+        for (int k = 0; k < NPASS; ++k)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                nxt[k][v] = __builtin_bit_cast(
+                    double, __builtin_amdgcn_raw_buffer_load_b64(
+                                rU, cell[k] * 16 + (jj & 1) * 8, (4 * (jj >> 1) + v) * g.C * 16,
+                                0));
+    };
+    fetch(j0);
+    for (int j = j0; j < j1; ++j) {
+        const int d = j >> 1, part = j & 1;
+#pragma unroll
+        for (int k = 0; k < NPASS; ++k) {
+            const int i = tid + k * NT;
+            if (i >= BOX) continue;
+            double du[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) du[v] = nxt[k][v];
+            if (kind[k] == 1) {
+                const int bnn = i / BM;
+                const double w0 = sW[bnn], w1 = sW[BN + bnn];
+                const double s = w0 * (du[1] - sW[2 * BN + bnn] * du[0]) +
+                                 w1 * (du[2] - sW[3 * BN + bnn] * du[0]);
+                const double t = 2.0 * s * sW[4 * BN + bnn];
+                du[1] = du[1] - w0 * t;
+                du[2] = du[2] - w1 * t;
+            } else if (kind[k] == 2) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) du[v] = -du[v];
+            } else if (kind[k] == 3) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) du[v] = 0.0;
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sdU[v * BOX + i] = du[v];
+            sdP[i] = (kGamma - 1.0) * (du[3] - cu[k] * du[1] - cv[k] * du[2] + ck[k] * du[0]);
+        }
+        __syncthreads();
+        if (j + 1 < j1) fetch(j + 1);
+
+        auto apply = [&](const FaceLin& f, int bLL, int bL, int bR, int bRR, auto outL,
+                         auto outR) {
+            const double dp[4] = {sdP[bLL], sdP[bL], sdP[bR], sdP[bRR]};
+            face_lin_apply(P, f, prim(bLL, bL, bR, bRR), sP[bL], sP[bR], sI[bL], sI[bR],
+                           dirs(bLL, bL, bR, bRR), dp, outL, outR);
+        };
+        double S[4] = {0.0, 0.0, 0.0, 0.0}, fb[4] = {0.0, 0.0, 0.0, 0.0},
+               fo[4] = {0.0, 0.0, 0.0, 0.0};
+        const int cw = tn % CPW;
+        if (live) {
+            apply(rf, fLL, bc, fR, fRR, [&](int v, double x) { S[v] = x; },
+                  [&](int v, double x) { fb[v] = x; });
+            if (cw == CPW - 1 && tn + 1 < TN && n + 1 < g.N)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) sFb[v * SB + (tn + 1) / CPW * TM + tm] = fb[v];
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const double x = __shfl_up(fb[v], TM, 64);
+            if (live && cw > 0) S[v] = S[v] + x;
+        }
+        if (live)
+            apply(ro, oLL, bc, oR, oRR, [&](int v, double x) { S[v] = S[v] + x; },
+                  [&](int v, double x) { fo[v] = x; });
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const double x = __shfl_up(fo[v], 1, 64);
+            if (live && tm > 0) S[v] = S[v] + x;
+        }
+        if (tid < NE && e_live) {
+            const FaceLin fe = lin_get(sRec, NE, tid);
+            double* dst = tid < TM ? sFb + tid : sFi + (tid - TM);
+            const int stride = tid < TM ? SB : TN;
+            if (e_own)
+                apply(fe, eLL, eL, eR, eRR, [&](int v, double x) { dst[v * stride] = x; },
+                      [](int, double) {});
+            else
+                apply(fe, eLL, eL, eR, eRR, [](int, double) {},
+                      [&](int v, double x) { dst[v * stride] = x; });
+        }
+        __syncthreads();
+        if (live) {
+            if (cw == 0)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) S[v] = S[v] + sFb[v * SB + tn / CPW * TM + tm];
+            if (tm == 0)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) S[v] = S[v] + sFi[v * TN + tn];
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                buf_store(rR, -(S[v] * ia), cme * 16 + part * 8, (4 * d + v) * g.C * 16);
+        }
+    }
+}
+
+// Real directions [j0, j1) of item z: chunk z of the 2 ndir real directions, `per` each.
+template <int TM, int TN>
+__global__ void __launch_bounds__(TM * TN)
+k_face_dirs(pea_params P, FsDir fs, LdReal ld, const double* __restrict__ dU,
+            double* __restrict__ dR, int ndir, int per) {
+    const TileId t = batch_tile(gridDim.x, gridDim.y, gridDim.z);
+    const int j0 = t.bz * per, j1 = min(2 * ndir, j0 + per);
+    face_dirs_tile<TM, TN>(P, fs, ld, dU, dR, ndir, j0, j1, t.bx, t.by);
 }
 
 struct StoreRealB {
@@ -1716,11 +2094,9 @@ template <int TM, int TN, int NT>
 __global__ void __launch_bounds__(NT)
 k_face_vjp(pea_params P, FsDir fs, LdReal ld, const double* __restrict__ wseed,
            double* __restrict__ gU) {
-    const unsigned tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y,
-                                   gridDim.x * gridDim.y);   // XCD-contiguous tiles
-    const long z = (long)blockIdx.z * 8 * ld.g.C;            // seed z of a batch
-    face_vjp_tile<TM, TN, NT>(P, fs, ld, wseed + z, gU + z, (int)(tile % gridDim.x) * TM,
-                              (int)(tile / gridDim.x) * TN);
+    const TileId t = batch_tile(gridDim.x, gridDim.y, gridDim.z);
+    const long z = (long)t.bz * 8 * ld.g.C;   // seed bz of a batch
+    face_vjp_tile<TM, TN, NT>(P, fs, ld, wseed + z, gU + z, t.bx * TM, t.by * TN);
 }
 
 // Sums per-block complex partials in block order (deterministic).
@@ -2334,6 +2710,43 @@ int pea_jvp_odes(const pea_params* prm, double mach, int M, int N, const double*
             *prm, fs_dir(*prm), LdDual<2>{X, V, U, dX, dV, dU, dmach, mach, g},
             StoreDualB<2>{dR, R, g.C});
     PEA_LAUNCH_CHECK("k_face_fwd");
+    return PEA_OK;
+}
+
+int pea_jvp_odes_states_dirs(const pea_params* prm, double mach, int M, int N, int k,
+                             const double* X, const double* V, const double* U, const double* dU,
+                             double* dR, int per, void* stream) {
+    if (int rc = check_dims(M, N)) return rc;
+    if (!prm || !X || !V || !U || !dU || !dR) return fail(PEA_ERR_NULL, "null argument");
+    if (k < 1) return fail(PEA_ERR_ARG, "k must be >= 1 (got %d)", k);
+    if (per < 0) return fail(PEA_ERR_ARG, "per must be >= 0 (got %d)", per);
+    const Grid g = make_grid(M, N);
+    constexpr int TM = 16, TN = 16;
+    const long tiles = (long)((M + TM - 1) / TM) * ((N + TN - 1) / TN);
+    // The buffer descriptors hold byte counts below 2^31: at most kmax complex directions per
+    // launch.
+    const int kmax = (int)std::max(1LL, ((1LL << 31) - 1) / (64LL * g.C));
+    hipStream_t st = as_stream(stream);
+    for (int d0 = 0; d0 < k; d0 += kmax) {
+        const int kd = std::min(kmax, k - d0);
+        // default: all of a launch's directions in each workgroup when the tiles alone fill the
+        // GPU two workgroups per CU twice; else split them over enough workgroups to do so
+        int pr = per;
+        if (pr == 0) {
+            const long want = 2L * 2 * 256;
+            const long chunks = std::max(1L, std::min((long)2 * kd, (want + tiles - 1) / tiles));
+            pr = (int)((2L * kd + chunks - 1) / chunks);
+        }
+        pr = std::min(pr, 2 * kd);
+        const int nz = (2 * kd + pr - 1) / pr;
+        if (nz > 65535) return fail(PEA_ERR_ARG, "too many direction chunks (%d)", nz);
+        const dim3 grid((M + TM - 1) / TM, (N + TN - 1) / TN, nz);
+        k_face_dirs<TM, TN><<<grid, TM * TN, 0, st>>>(*prm, fs_dir(*prm),
+                                                        LdReal{X, V, U, mach, g},
+                                                        dU + (long)d0 * 8 * g.C,
+                                                        dR + (long)d0 * 8 * g.C, kd, pr);
+        PEA_LAUNCH_CHECK("k_face_dirs");
+    }
     return PEA_OK;
 }
 

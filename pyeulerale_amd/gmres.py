@@ -6,7 +6,7 @@ The reference solves one shift at a time with SuperLU on a freshly assembled CSR
 * operator: ``"blocks"`` -- the assembled complex-step blocks applied to all S vectors in one
   launch (``pea_block_multi``, blocks read once per batch); or ``"jvp"`` -- matrix-free, the
   hand-written tangent (adjoint for the transposed system) of all S vectors in one batched
-  launch (``pea_jvp_odes_multi`` / ``pea_vjp_odes_states_multi``), no assembled matrix;
+  launch (``pea_jvp_odes_states_dirs`` / ``pea_vjp_odes_states_multi``), no assembled matrix;
 * right preconditioning per shift: ``"line"`` (block LU of every radial line,
   ``pea_line_factor``/``pea_line_solve``), ``"jacobi"`` (4x4 diagonal blocks), ``"nested"`` (the
   exact nested-dissection factorisation of one anchor shift for every shift of the batch, all
@@ -124,13 +124,14 @@ class BatchedGMRES:
             _lib.call("pea_block_multi", g.M, g.N, self.S, _p(self.JU), _p(self.sh),
                       int(self.adjoint), _p(x), _p(y), K._stream())
             return y
-        # matrix-free: all S products in one batched launch of the face tangent (adjoint), then
-        # y -= s x with s per vector
+        # matrix-free: all S products in one launch, then y -= s x with s per vector.  The
+        # tangent evaluates each tile's primal part once for all S directions
+        # (pea_jvp_odes_states_dirs); the adjoint runs one seed per grid z.
         p, mach, gg, X, V, U = self.point
         if self.adjoint:
             K.vjp_odes_states_multi(p, mach, g, X, V, U, x, gU=y)
         else:
-            K.jvp_odes_multi(p, mach, g, X, V, U, dU=x, dR=y)
+            K.jvp_odes_dirs(p, mach, g, X, V, U, x, dR=y)
         y.addcmul_(x, self._neg_shift)
         return y
 

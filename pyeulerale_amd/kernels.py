@@ -268,6 +268,22 @@ def jvp_odes_multi(p: Params, mach: float, g: Grid, X, V, U, dU=None, dX=None, d
     return dR
 
 
+def jvp_odes_dirs(p: Params, mach: float, g: Grid, X, V, U, dU, dR=None, per: int = 0
+                  ) -> torch.Tensor:
+    """(dR/dU) dU[j] for k complex directions (dU of shape (k, 4, N, M)) at one state: each
+    workgroup evaluates its tile's primal part once and applies the linearised face fluxes to
+    `per` real directions (0: chosen from the grid size).  The same products as
+    jvp_odes_multi(..., dU=dU), to round-off."""
+    g.check_primal(X, V, U)
+    k = dU.shape[0]
+    _need(dU, (k, *g.cell_shape), C128, "d_states")
+    dR = torch.empty((k, *g.cell_shape), dtype=C128, device=U.device) if dR is None else \
+        _need(dR, (k, *g.cell_shape), C128, "d_odes")
+    _lib.call("pea_jvp_odes_states_dirs", ctypes.byref(p.c()), float(mach), g.M, g.N, k,
+              _ptr(X), _ptr(V), _ptr(U), _ptr(dU), _ptr(dR), int(per), _stream())
+    return dR
+
+
 def vjp_odes_states_multi(p: Params, mach: float, g: Grid, X, V, U, w, gU=None) -> torch.Tensor:
     """(dR/dU)^T w for k seeds at once (w of shape (k, 4, N, M), complex): one batched launch of
     the face adjoint."""
