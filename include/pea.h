@@ -153,9 +153,8 @@ int pea_jvp_odes_multi(const pea_params *prm, double mach, int M, int N, int k,
 /* dR[j] = dR/dU dU[j] for k complex directions at one state (dU, dR: k consecutive complex
  * cell fields), the same products as pea_jvp_odes_multi along dU alone.  Each workgroup computes
  * its tile's primal part once (states, pressures, and per face the switch decisions and
- * coefficients of the dissipation) and applies it to `per` real directions (the real and
- * imaginary parts of the complex ones; per = 0: chosen from the grid size), so the primal flux is
- * not evaluated again per direction.  Agrees with pea_jvp_odes_multi to round-off.  The
+ * coefficients of the dissipation) and applies it to `per` complex directions (per = 0: chosen
+ * from the grid size), so the primal flux is not evaluated again per direction.  Agrees with pea_jvp_odes_multi to round-off.  The
  * reference's products apply assembled complex-step blocks per direction
  * (apply_odes_wrt_states_fwd, f90:825-864). */
 int pea_jvp_odes_states_dirs(const pea_params *prm, double mach, int M, int N, int k,

@@ -764,11 +764,11 @@ k_face_fwd(pea_params P, FsDir fs, Ld ld, Store store) {
 // per tile, as face_fwd_tile, but the primal part is done once per tile: the states, pressures and
 // 1/rho of the grown tile go to LDS, and every face the tile evaluates gets its FaceLin record
 // (pea_linear.hpp): in registers for the f and o faces a thread owns, in LDS for the tile-edge
-// faces of the first wave.  Then the workgroup walks real directions j0 .. j1-1 (the real and
-// imaginary parts of complex directions j/2): it stages the direction's states of the grown tile
-// and their pressure tangents in LDS, applies each face's record, hands the other side's flux to
-// the neighbouring cell as face_fwd_tile does, and stores -(sum of the four face fluxes)/area.
-// Every global access is a buffer access bounded by its array.
+// faces of the first wave.  Then the workgroup walks complex directions d0 .. d1-1: it stages a
+// direction's states of the grown tile (real and imaginary parts) and their pressure tangents in
+// LDS, applies each face's record to both parts, hands the other side's flux to the neighbouring
+// cell as face_fwd_tile does, and stores -(sum of the four face fluxes)/area.  Every global
+// access is a buffer access bounded by its array.
 __device__ __forceinline__ void lin_put(double* base, int stride, int i, const FaceLin& f) {
     const double w[kLinWords] = {f.n0,   f.n1,   f.hvn,  f.L.lam, f.L.s, f.L.ib, f.L.aC,
                                  f.L.bC, f.R.lam, f.R.s, f.R.ib, f.R.aC, f.R.bC,
@@ -787,17 +787,18 @@ __device__ __forceinline__ FaceLin lin_get(const double* base, int stride, int i
     f.bits = (unsigned)__builtin_bit_cast(unsigned long long, w[13]);
     return f;
 }
-typedef unsigned int pea_u32x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, double x, int byte_off,
-                                          int plane_bytes) {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(pea_u32x2, x), r, byte_off,
-                                          plane_bytes, 0);
+typedef unsigned int pea_u32x4 __attribute__((ext_vector_type(4)));
+// complex element `elem` of plane `plane_elems` (in complex elements) of an interleaved array
+__device__ __forceinline__ void buf_store_cx(__amdgpu_buffer_rsrc_t r, double2 x, int elem,
+                                             int plane_elems) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(pea_u32x4, x), r, elem * 16,
+                                           plane_elems * 16, 0);
 }
 
 template <int TM, int TN>
 __device__ __forceinline__ void face_dirs_tile(const pea_params& P, const FsDir& fs,
                                                const LdReal& l, const double* dU, double* dR,
-                                               int ndir, int j0, int j1, int bx, int by) {
+                                               int ndir, int d0, int d1, int bx, int by) {
     constexpr int NT = TM * TN, BM = TM + 4, BN = TN + 4, BOX = BM * BN;
     static_assert(TM + TN <= NT, "the tile-edge faces take one thread each");
     static_assert(64 % TM == 0 && NT % 64 == 0, "a wave holds whole tile columns");
@@ -807,10 +808,10 @@ __device__ __forceinline__ void face_dirs_tile(const pea_params& P, const FsDir&
     // pressure tangents (sdP holds the primal sound speeds while the records are built); the
     // tile-edge records; the wall-ghost map of each box column; the flux hand-offs.
     __shared__ double sU[4 * BOX], sP[BOX], sI[BOX];
-    __shared__ double sdU[4 * BOX], sdP[BOX];
+    __shared__ double sdU[2 * 4 * BOX], sdP[2 * BOX];
     __shared__ double sRec[kLinWords * NE];
     __shared__ double sW[5 * BN];
-    __shared__ double sFb[4 * SB], sFi[4 * TN];
+    __shared__ double sFb[2 * 4 * SB], sFi[2 * 4 * TN];
     double* const sC = sdP;
     const Grid g = l.g;
     const int m0 = bx * TM, n0 = by * TN;
@@ -1002,119 +1003,132 @@ __device__ __forceinline__ void face_dirs_tile(const pea_params& P, const FsDir&
     }
     __syncthreads();
 
-    // 3. directions.  Loads and stores address one real part of the interleaved complex arrays:
-    // element (d, v, cell) part j & 1 at byte 16 ((4 d + v) C + cell) + 8 (j & 1).
+    // 3. directions.  Complex direction d is staged whole, its real part in the first half of
+    // sdU / sdP and its imaginary part in the second (one 16-byte load per element, at byte
+    // 16 ((4 d + v) C + cell)); every face is applied to the real part, then to the imaginary
+    // part, and the product is stored as whole complex elements.
     const auto rU = buf_rsrc(dU, 64L * g.C * ndir);
     const auto rR = buf_rsrc(dR, 64L * g.C * ndir);
     const int cme = n * g.M + m;
-    // the direction's states of this thread's box cells, loaded one direction ahead so that
-    // the loads overlap the previous direction's face evaluations
-    double nxt[NPASS][4];
-    auto fetch = [&](int jj) {
-#pragma unroll
-        for (int k = 0; k < NPASS; ++k)
-#pragma unroll
-            for (int v = 0; v < 4; ++v)
-                nxt[k][v] = __builtin_bit_cast(
-                    double, __builtin_amdgcn_raw_buffer_load_b64(
-                                rU, cell[k] * 16 + (jj & 1) * 8, (4 * (jj >> 1) + v) * g.C * 16,
-                                0));
-    };
-    fetch(j0);
-    for (int j = j0; j < j1; ++j) {
-        const int d = j >> 1, part = j & 1;
+    const int cw = tn % CPW;
+    for (int d = d0; d < d1; ++d) {
 #pragma unroll
         for (int k = 0; k < NPASS; ++k) {
             const int i = tid + k * NT;
             if (i >= BOX) continue;
-            double du[4];
+            double2 z[4];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) du[v] = nxt[k][v];
-            if (kind[k] == 1) {
-                const int bnn = i / BM;
-                const double w0 = sW[bnn], w1 = sW[BN + bnn];
-                const double s = w0 * (du[1] - sW[2 * BN + bnn] * du[0]) +
-                                 w1 * (du[2] - sW[3 * BN + bnn] * du[0]);
-                const double t = 2.0 * s * sW[4 * BN + bnn];
-                du[1] = du[1] - w0 * t;
-                du[2] = du[2] - w1 * t;
-            } else if (kind[k] == 2) {
+            for (int v = 0; v < 4; ++v) z[v] = buf_load_cx(rU, cell[k], (4 * d + v) * g.C);
 #pragma unroll
-                for (int v = 0; v < 4; ++v) du[v] = -du[v];
-            } else if (kind[k] == 3) {
+            for (int part = 0; part < 2; ++part) {
+                double du[4];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) du[v] = 0.0;
+                for (int v = 0; v < 4; ++v) du[v] = part ? z[v].y : z[v].x;
+                if (kind[k] == 1) {
+                    const int bnn = i / BM;
+                    const double w0 = sW[bnn], w1 = sW[BN + bnn];
+                    const double sg = w0 * (du[1] - sW[2 * BN + bnn] * du[0]) +
+                                      w1 * (du[2] - sW[3 * BN + bnn] * du[0]);
+                    const double tt = 2.0 * sg * sW[4 * BN + bnn];
+                    du[1] = du[1] - w0 * tt;
+                    du[2] = du[2] - w1 * tt;
+                } else if (kind[k] == 2) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) du[v] = -du[v];
+                } else if (kind[k] == 3) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) du[v] = 0.0;
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) sdU[(part * 4 + v) * BOX + i] = du[v];
+                sdP[part * BOX + i] =
+                    (kGamma - 1.0) * (du[3] - cu[k] * du[1] - cv[k] * du[2] + ck[k] * du[0]);
+            }
+        }
+        __syncthreads();
+
+        double S2[2][4];
+#pragma unroll
+        for (int part = 0; part < 2; ++part) {
+            const double* dUp = sdU + part * 4 * BOX;
+            const double* dPp = sdP + part * BOX;
+            double* Fb = sFb + part * 4 * SB;
+            double* Fi = sFi + part * 4 * TN;
+            auto dirs = [dUp](int bLL, int bL, int bR, int bRR) {
+                return [dUp, bLL, bL, bR, bRR](int slot, int v) -> double {
+                    const int b = slot == 0 ? bLL : slot == 1 ? bL : slot == 2 ? bR : bRR;
+                    return dUp[v * BOX + b];
+                };
+            };
+            auto apply = [&](const FaceLin& f, int bLL, int bL, int bR, int bRR, auto outL,
+                             auto outR) {
+                const double dp[4] = {dPp[bLL], dPp[bL], dPp[bR], dPp[bRR]};
+                face_lin_apply(P, f, prim(bLL, bL, bR, bRR), sP[bL], sP[bR], sI[bL], sI[bR],
+                               dirs(bLL, bL, bR, bRR), dp, outL, outR);
+            };
+            double* S = S2[part];
+            double fb[4] = {0.0, 0.0, 0.0, 0.0}, fo[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int v = 0; v < 4; ++v) S[v] = 0.0;
+            if (live) {
+                apply(rf, fLL, bc, fR, fRR, [&](int v, double x) { S[v] = x; },
+                      [&](int v, double x) { fb[v] = x; });
+                if (cw == CPW - 1 && tn + 1 < TN && n + 1 < g.N)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) Fb[v * SB + (tn + 1) / CPW * TM + tm] = fb[v];
             }
 #pragma unroll
-            for (int v = 0; v < 4; ++v) sdU[v * BOX + i] = du[v];
-            sdP[i] = (kGamma - 1.0) * (du[3] - cu[k] * du[1] - cv[k] * du[2] + ck[k] * du[0]);
-        }
-        __syncthreads();
-        if (j + 1 < j1) fetch(j + 1);
-
-        auto apply = [&](const FaceLin& f, int bLL, int bL, int bR, int bRR, auto outL,
-                         auto outR) {
-            const double dp[4] = {sdP[bLL], sdP[bL], sdP[bR], sdP[bRR]};
-            face_lin_apply(P, f, prim(bLL, bL, bR, bRR), sP[bL], sP[bR], sI[bL], sI[bR],
-                           dirs(bLL, bL, bR, bRR), dp, outL, outR);
-        };
-        double S[4] = {0.0, 0.0, 0.0, 0.0}, fb[4] = {0.0, 0.0, 0.0, 0.0},
-               fo[4] = {0.0, 0.0, 0.0, 0.0};
-        const int cw = tn % CPW;
-        if (live) {
-            apply(rf, fLL, bc, fR, fRR, [&](int v, double x) { S[v] = x; },
-                  [&](int v, double x) { fb[v] = x; });
-            if (cw == CPW - 1 && tn + 1 < TN && n + 1 < g.N)
+            for (int v = 0; v < 4; ++v) {
+                const double x = __shfl_up(fb[v], TM, 64);
+                if (live && cw > 0) S[v] = S[v] + x;
+            }
+            if (live)
+                apply(ro, oLL, bc, oR, oRR, [&](int v, double x) { S[v] = S[v] + x; },
+                      [&](int v, double x) { fo[v] = x; });
 #pragma unroll
-                for (int v = 0; v < 4; ++v) sFb[v * SB + (tn + 1) / CPW * TM + tm] = fb[v];
-        }
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const double x = __shfl_up(fb[v], TM, 64);
-            if (live && cw > 0) S[v] = S[v] + x;
-        }
-        if (live)
-            apply(ro, oLL, bc, oR, oRR, [&](int v, double x) { S[v] = S[v] + x; },
-                  [&](int v, double x) { fo[v] = x; });
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const double x = __shfl_up(fo[v], 1, 64);
-            if (live && tm > 0) S[v] = S[v] + x;
-        }
-        if (tid < NE && e_live) {
-            const FaceLin fe = lin_get(sRec, NE, tid);
-            double* dst = tid < TM ? sFb + tid : sFi + (tid - TM);
-            const int stride = tid < TM ? SB : TN;
-            if (e_own)
-                apply(fe, eLL, eL, eR, eRR, [&](int v, double x) { dst[v * stride] = x; },
-                      [](int, double) {});
-            else
-                apply(fe, eLL, eL, eR, eRR, [](int, double) {},
-                      [&](int v, double x) { dst[v * stride] = x; });
+            for (int v = 0; v < 4; ++v) {
+                const double x = __shfl_up(fo[v], 1, 64);
+                if (live && tm > 0) S[v] = S[v] + x;
+            }
+            if (tid < NE && e_live) {
+                const FaceLin fe = lin_get(sRec, NE, tid);
+                double* dst = tid < TM ? Fb + tid : Fi + (tid - TM);
+                const int stride = tid < TM ? SB : TN;
+                if (e_own)
+                    apply(fe, eLL, eL, eR, eRR, [&](int v, double x) { dst[v * stride] = x; },
+                          [](int, double) {});
+                else
+                    apply(fe, eLL, eL, eR, eRR, [](int, double) {},
+                          [&](int v, double x) { dst[v * stride] = x; });
+            }
         }
         __syncthreads();
         if (live) {
-            if (cw == 0)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) S[v] = S[v] + sFb[v * SB + tn / CPW * TM + tm];
-            if (tm == 0)
+            for (int v = 0; v < 4; ++v) {
+                double2 r;
+                double* rp = reinterpret_cast<double*>(&r);
 #pragma unroll
-                for (int v = 0; v < 4; ++v) S[v] = S[v] + sFi[v * TN + tn];
-#pragma unroll
-            for (int v = 0; v < 4; ++v)
-                buf_store(rR, -(S[v] * ia), cme * 16 + part * 8, (4 * d + v) * g.C * 16);
+                for (int part = 0; part < 2; ++part) {
+                    double x = S2[part][v];
+                    if (cw == 0) x = x + sFb[(part * 4 + v) * SB + tn / CPW * TM + tm];
+                    if (tm == 0) x = x + sFi[(part * 4 + v) * TN + tn];
+                    rp[part] = -(x * ia);
+                }
+                buf_store_cx(rR, r, cme, (4 * d + v) * g.C);
+            }
         }
     }
 }
 
-// Real directions [j0, j1) of item z: chunk z of the 2 ndir real directions, `per` each.
+// Complex directions [d0, d1) of item z: chunk z of the ndir directions, `per` each.
 template <int TM, int TN>
 __global__ void __launch_bounds__(TM * TN)
 k_face_dirs(pea_params P, FsDir fs, LdReal ld, const double* __restrict__ dU,
             double* __restrict__ dR, int ndir, int per) {
     const TileId t = batch_tile(gridDim.x, gridDim.y, gridDim.z);
-    const int j0 = t.bz * per, j1 = min(2 * ndir, j0 + per);
-    face_dirs_tile<TM, TN>(P, fs, ld, dU, dR, ndir, j0, j1, t.bx, t.by);
+    const int d0 = t.bz * per, d1 = min(ndir, d0 + per);
+    face_dirs_tile<TM, TN>(P, fs, ld, dU, dR, ndir, d0, d1, t.bx, t.by);
 }
 
 struct StoreRealB {
@@ -2734,11 +2748,11 @@ int pea_jvp_odes_states_dirs(const pea_params* prm, double mach, int M, int N, i
         int pr = per;
         if (pr == 0) {
             const long want = 2L * 2 * 256;
-            const long chunks = std::max(1L, std::min((long)2 * kd, (want + tiles - 1) / tiles));
-            pr = (int)((2L * kd + chunks - 1) / chunks);
+            const long chunks = std::max(1L, std::min((long)kd, (want + tiles - 1) / tiles));
+            pr = (int)((kd + chunks - 1) / chunks);
         }
-        pr = std::min(pr, 2 * kd);
-        const int nz = (2 * kd + pr - 1) / pr;
+        pr = std::min(pr, kd);
+        const int nz = (kd + pr - 1) / pr;
         if (nz > 65535) return fail(PEA_ERR_ARG, "too many direction chunks (%d)", nz);
         const dim3 grid((M + TM - 1) / TM, (N + TN - 1) / TN, nz);
         k_face_dirs<TM, TN><<<grid, TM * TN, 0, st>>>(*prm, fs_dir(*prm),

@@ -272,7 +272,7 @@ def jvp_odes_dirs(p: Params, mach: float, g: Grid, X, V, U, dU, dR=None, per: in
                   ) -> torch.Tensor:
     """(dR/dU) dU[j] for k complex directions (dU of shape (k, 4, N, M)) at one state: each
     workgroup evaluates its tile's primal part once and applies the linearised face fluxes to
-    `per` real directions (0: chosen from the grid size).  The same products as
+    `per` of the directions (0: chosen from the grid size).  The same products as
     jvp_odes_multi(..., dU=dU), to round-off."""
     g.check_primal(X, V, U)
     k = dU.shape[0]

@@ -67,13 +67,13 @@ def test_against_dual_tangent_and_oracle(name):
 
 
 def test_direction_chunks_are_bitwise_alike():
-    """Every split of the real directions over workgroups gives the same bits: a direction's
+    """Every split of the directions over workgroups gives the same bits: a direction's
     arithmetic does not depend on which workgroup applies it."""
-    k = 3
+    k = 5
     K, g, p, X, V, U, Xd, Vd, Ud, dU = setup("ragged", k)
     dUd = torch.stack([dev_cells(d) for d in dU])
-    base = K.jvp_odes_dirs(p, 0.5, g, Xd, Vd, Ud, dUd, per=2 * k)
-    for per in (0, 1, 2, 3, 4, 5):
+    base = K.jvp_odes_dirs(p, 0.5, g, Xd, Vd, Ud, dUd, per=k)
+    for per in (0, 1, 2, 3, 4, 6):
         assert torch.equal(K.jvp_odes_dirs(p, 0.5, g, Xd, Vd, Ud, dUd, per=per), base), per
     # one direction alone equals the same direction within a batch
     one = K.jvp_odes_dirs(p, 0.5, g, Xd, Vd, Ud, dUd[1:2].contiguous())

@@ -139,4 +139,14 @@ def test_multifrontal_reproduces_dense_solve(M, N, leaf, adjoint):
 def test_factor_memory_at_1025x513_fits_one_gpu():
     st = symbolic(512, 1024)
     assert 40e9 < st.factor_bytes() < 70e9        # vs 412 GB for the ring LU
-    assert 60e12 < st.flops() < 110e12
+    assert 30e12 < st.flops() < 55e12
+
+
+def test_flop_count_of_one_front():
+    """A grid that is a single leaf has one front (its 4 C unknowns, no ring): the count is then
+    the textbook one of a complex LU, 8/3 n^3 real operations (a complex multiply-add is 8)."""
+    st = symbolic(4, 8, leaf=64)
+    assert len(st.levels) == 1 and len(st.levels[0].nodes) == 1 and st.levels[0].B == 0
+    n = st.levels[0].P
+    assert n == 4 * 4 * 8
+    assert st.flops() == 8.0 / 3.0 * n ** 3
