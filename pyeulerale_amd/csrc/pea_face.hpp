@@ -106,6 +106,47 @@ PEA_HD void face_flux2_lds(const pea_params& P, const St& st, const T& pLL, cons
     }
 }
 
+// face_flux2_lds in FP64 with per-cell quantities computed once per cell instead of in every
+// face that reads them: iL, iR = 1/rho of L and R as rcp_fast gives them (so q = (n.j) iL is
+// the same value qdiv forms), nuL, nuR = the pressure sensors of L and R along the face's grid
+// line, |p_prev - 2 p + p_next| / (p_prev + 2 p + p_next) with prev the lower index.  L's side
+// reads its sensors in the reference's order (f90:209-210) and gets them bitwise; R's side
+// (f90 evaluates sensor(rr, r, l) and sensor(r, l, ll)) gets the same triples summed from the
+// other end, which can differ by one rounding.
+template <class St, class OutL, class OutR>
+PEA_HD void face_flux2_cell(const pea_params& P, const St& st, double pL, double pR, double cL,
+                            double cR, double iL, double iR, double nuL, double nuR,
+                            const double* n, const double* gv, bool dissL, bool dissR,
+                            OutL outL, OutR outR) {
+    const double qL = (n[0] * st(1, 1) + n[1] * st(1, 2)) * iL;
+    const double qR = (n[0] * st(2, 1) + n[1] * st(2, 2)) * iR;
+    const double vn = gv[0] * n[0] + gv[1] * n[1];
+    double e2L = 0.0, e4L = 0.0, e2R = 0.0, e4R = 0.0;
+    if (dissL || dissR) {
+        const double sn = sqrt_pos(n[0] * n[0] + n[1] * n[1]);
+        const double lamL = fabs(qL) + cL * sn, lamR = fabs(qR) + cR * sn;
+        if (dissL) side_coefs(P, lamL, lamR, nuL, nuR, e2L, e4L);
+        if (dissR) side_coefs(P, lamR, lamL, nuR, nuL, e2R, e4R);
+    }
+    const double hvn = vn / 2.0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const double Lv = st(1, v), Rv = st(2, v);
+        const double phL = v == 0 ? Lv * qL : v == 3 ? (Lv + pL) * qL : Lv * qL + n[v - 1] * pL;
+        const double phR = v == 0 ? Rv * qR : v == 3 ? (Rv + pR) * qR : Rv * qR + n[v - 1] * pR;
+        const double cen = (phL + phR) / 2.0;
+        const double ale = (Lv + Rv) * hvn;
+        double a = cen, b = -cen;
+        if (dissL || dissR) {
+            const double d2 = Rv - Lv, d4 = st(3, v) - 3.0 * Rv + 3.0 * Lv - st(0, v);
+            if (dissL) a = a - (e2L * d2 - e4L * d4);
+            if (dissR) b = b + (e2R * d2 - e4R * d4);
+        }
+        outL(v, a - ale);
+        outR(v, b + ale);
+    }
+}
+
 // ---------------------------------------------------------------------------- reverse
 // Reverse of side_dissipation with seed db on D: accumulates into the adjoints of the side's
 // ll, l, r, rr states (sb*), of its two wave integrals (lamb_l, lamb_r) and of the four

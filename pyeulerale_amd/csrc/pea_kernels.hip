@@ -476,6 +476,11 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const FsDir& 
     constexpr int W = sizeof(T) / sizeof(double);
     __shared__ T sP[BOX], sC[BOX];
     __shared__ double sU[4 * W * BOX];
+    // FP64: 1/rho and the pressure sensors along m and n of each box cell, computed once per
+    // cell (face_flux2_cell); the tangent kernels evaluate them per face (face_flux2_lds)
+    constexpr bool kCell = std::is_same<T, double>::value;
+    constexpr int CB = kCell ? BOX : 1;
+    __shared__ double sI[CB], sNm[CB], sNn[CB];
     __shared__ double sFb[4 * W * SB], sFi[4 * W * TN];
     const Grid g = ld.g;
     const Ld l = ld.at(bz);
@@ -562,6 +567,7 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const FsDir& 
         const int cn = wrap_any(n0 - 2 + bn, g.N);
         if (cm >= 0 && cm < g.M) {
             cell_pc(u, sP[i], sC[i]);
+            if constexpr (kCell) sI[i] = rcp_fast(val(u[0]));
 #pragma unroll
             for (int v = 0; v < 4; ++v) lds_put(sU + v * W * BOX, BOX, i, u[v]);
         } else if (cm == -1 || cm == g.M) {
@@ -601,6 +607,18 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const FsDir& 
         }
     }
     __syncthreads();
+    if constexpr (kCell) {
+        // sensors of the cells the tile's faces read as L or R (box interior; rows outside
+        // [-1, M] hold no pressure and their sensors are never read)
+        for (int i = tid; i < BOX; i += NT) {
+            const int bm = i % BM, bn = i / BM;
+            if (bm < 1 || bm > BM - 2 || bn < 1 || bn > BN - 2) continue;
+            const double p = val(sP[i]);
+            sNm[i] = sensor(val(sP[i - 1]), p, val(sP[i + 1]));
+            sNn[i] = sensor(val(sP[i - BM]), p, val(sP[i + BM]));
+        }
+        __syncthreads();
+    }
 
     auto bidx = [&](int mm, int nb) { return nb * BM + (mm - m0 + 2); };   // nb: box column
     // state of cell (mm, box column nb); mm = -1 and mm = M are the ghosts
@@ -616,6 +634,17 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const FsDir& 
             const int b = slot == 0 ? bLL : slot == 1 ? bL : slot == 2 ? bR : bRR;
             return lds_get<T>(u + v * W * BOX, BOX, b);
         };
+    };
+    // both fluxes of the face L | R (box cells bLL, bL, bR, bRR along one grid line; nu: the
+    // sensors along that line)
+    auto flux2 = [&](int bLL, int bL, int bR, int bRR, const double* nu, const G* nr,
+                     const G* nr2, const G* gv, bool dL, bool dR, auto outL, auto outR) {
+        if constexpr (kCell)
+            face_flux2_cell(P, stencil(bLL, bL, bR, bRR), sP[bL], sP[bR], sC[bL], sC[bR], sI[bL],
+                            sI[bR], nu[bL], nu[bR], nr, gv, dL, dR, outL, outR);
+        else
+            face_flux2_lds(P, stencil(bLL, bL, bR, bRR), sP[bLL], sP[bL], sP[bR], sP[bRR],
+                           sC[bL], sC[bR], nr, nr2, gv, dL, dR, outL, outR);
     };
     T S[4];       // running sum of this cell's four face fluxes
     T fb[4], fo[4];   // fluxes of this cell's f and o faces for the cells at n + 1 and m + 1
@@ -634,10 +663,8 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const FsDir& 
         edge_of(gx[FI], gx[FO], gw[0], gw[1], nr, gv);   // fi -> fo
         edge_normal(gx[FO], gx[FI], nr2);                // R's bo -> bi
         const int bLL = bidx(m, bn - 1), bR = bidx(m, bn + 1), bRR = bidx(m, bn + 2);
-        face_flux2_lds(P, stencil(bLL, bc, bR, bRR), sP[bLL], sP[bc], sP[bR], sP[bRR],
-                       sC[bc], sC[bR], nr, nr2, gv, true, true,
-                       [&](int v, const T& f) { S[v] = f; },
-                       [&](int v, const T& f) { fb[v] = f; });
+        flux2(bLL, bc, bR, bRR, sNn, nr, nr2, gv, true, true,
+              [&](int v, const T& f) { S[v] = f; }, [&](int v, const T& f) { fb[v] = f; });
         // the cell at n + 1 starts the next wave: hand over through LDS
         if (cw == CPW - 1 && tn + 1 < TN && n + 1 < g.N)
 #pragma unroll
@@ -665,10 +692,9 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const FsDir& 
             edge_normal(gx[BO], gx[FO], nr2);                // R's bi -> fi
             // m - 1 is the wall ghost row for m = 0, m + 2 the far-field row for m + 1 = M - 1
             const int bLL = bidx(m - 1, bn), bR = bidx(m + 1, bn), bRR = bidx(m + 2, bn);
-            face_flux2_lds(P, stencil(bLL, bc, bR, bRR), sP[bLL], sP[bc], sP[bR], sP[bRR],
-                           sC[bc], sC[bR], nr, nr2, gv, m != g.M - 2, m + 1 != 1,
-                           [&](int v, const T& f) { S[v] = S[v] + f; },
-                           [&](int v, const T& f) { fo[v] = f; });
+            flux2(bLL, bc, bR, bRR, sNm, nr, nr2, gv, m != g.M - 2, m + 1 != 1,
+                  [&](int v, const T& f) { S[v] = S[v] + f; },
+                  [&](int v, const T& f) { fo[v] = f; });
         } else {
             T uc[4], gh[4], Fo[4];
             box(m, bn, uc);
@@ -698,9 +724,8 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const FsDir& 
             edge_of(ex[0], ex[1], ew[0], ew[1], nr, gv);   // bi -> bo
             edge_normal(ex[1], ex[0], nr2);                // the cell's bo -> bi
             const int bLL = bidx(mm, 0), bL = bidx(mm, 1), bR = bidx(mm, 2), bRR = bidx(mm, 3);
-            face_flux2_lds(P, stencil(bLL, bL, bR, bRR), sP[bLL], sP[bL], sP[bR], sP[bRR],
-                           sC[bL], sC[bR], nr, nr2, gv, false, true, [](int, const T&) {},
-                           [&](int v, const T& f) { lds_put(sFb + v * W * SB, SB, tid, f); });
+            flux2(bLL, bL, bR, bRR, sNn, nr, nr2, gv, false, true, [](int, const T&) {},
+                  [&](int v, const T& f) { lds_put(sFb + v * W * SB, SB, tid, f); });
         }
     } else if (tid < TM + TN) {
         const int j = tid - TM, mm = m0, nn = n0 + j, bnn = j + 2;
@@ -723,10 +748,8 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const FsDir& 
                 // mm = M - 1
                 const int bLL = bidx(mm - 2, bnn), bL = bidx(mm - 1, bnn), bR = bidx(mm, bnn),
                           bRR = bidx(mm + 1, bnn);
-                face_flux2_lds(P, stencil(bLL, bL, bR, bRR), sP[bLL], sP[bL], sP[bR], sP[bRR],
-                               sC[bL], sC[bR], nr, nr2, gv, false, mm != 1,
-                               [](int, const T&) {},
-                               [&](int v, const T& f) { lds_put(sFi + v * W * TN, TN, j, f); });
+                flux2(bLL, bL, bR, bRR, sNm, nr, nr2, gv, false, mm != 1, [](int, const T&) {},
+                      [&](int v, const T& f) { lds_put(sFi + v * W * TN, TN, j, f); });
             }
         }
     }

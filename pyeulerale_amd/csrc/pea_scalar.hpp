@@ -90,6 +90,25 @@ PEA_HD double rcp_fast(double x) {
 #endif
 }
 
+// sqrt(x) for a positive normal x (an edge length from its squared components): the hardware
+// reciprocal square root estimate refined by one Goldschmidt step and one Newton correction,
+// within about 1 ulp of the correctly rounded root, without the range scaling and special-value
+// handling of IEEE sqrt; x == 0 gives 0.
+PEA_HD double sqrt_pos(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    const double r = __builtin_amdgcn_rsq(x);
+    double s = x * r, h = 0.5 * r;
+    const double e = fma(-h, s, 0.5);
+    s = fma(s, e, s);
+    h = fma(h, e, h);
+    const double d = fma(-s, s, x);
+    s = fma(d, h, s);
+    return x > 0.0 ? s : 0.0;
+#else
+    return sqrt(x);
+#endif
+}
+
 // ------------------------------------------------------------------------------- dual
 template <int K>
 struct Dual {
