@@ -12,6 +12,8 @@
 // coefficients, evaluated with exactly the reference's expressions.  Pressures and sound speeds
 // sqrt(gamma p / rho) are per-cell quantities computed once (same expressions as f90:128-147).
 #pragma once
+#include <type_traits>
+
 #include "pea_physics.hpp"
 
 namespace pea {

@@ -93,7 +93,7 @@ PEA_HD double rcp_fast(double x) {
 // sqrt(x) for a positive normal x (an edge length from its squared components): the hardware
 // reciprocal square root estimate refined by one Goldschmidt step and one Newton correction,
 // within about 1 ulp of the correctly rounded root, without the range scaling and special-value
-// handling of IEEE sqrt; x == 0 gives 0.
+// handling of IEEE sqrt.  x == 0 gives 0; a negative or NaN x gives NaN.
 PEA_HD double sqrt_pos(double x) {
 #ifdef __HIP_DEVICE_COMPILE__
     const double r = __builtin_amdgcn_rsq(x);
@@ -103,7 +103,7 @@ PEA_HD double sqrt_pos(double x) {
     h = fma(h, e, h);
     const double d = fma(-s, s, x);
     s = fma(d, h, s);
-    return x > 0.0 ? s : 0.0;
+    return x == 0.0 ? 0.0 : s;
 #else
     return sqrt(x);
 #endif
