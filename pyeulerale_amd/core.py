@@ -438,7 +438,9 @@ class SpatialDiscretization:
         factorisation per shift (``"ring"`` / ``"nested"`` choose).  ``solver="gmres"`` runs
         batched GMRES; with ``precond="nested"`` the shifts are split into ``groups`` groups by
         |s|, each preconditioned by the exact factorisation at its middle shift
-        (``last_groups``).  Shifts whose residual is still above ``rtol`` after ``maxiter`` are
+        (``last_groups``); ``groups="adaptive"`` forms the groups from the lowest unsolved |s|
+        up as convergence allows and solves the shifts of the lightly damped band one
+        factorisation each (``nested.solve_shifts_adaptive``; ``last_exact``).  Shifts whose residual is still above ``rtol`` after ``maxiter`` are
         solved again with the direct method when ``fallback="direct"`` (the default), or left as
         they are (``fallback=None``; ``last_gmres.converged`` tells which)."""
         t = d_odes if isinstance(d_odes, torch.Tensor) else _as_tensor(d_odes, C128, self.device)
@@ -459,6 +461,28 @@ class SpatialDiscretization:
         elif solver == "nested":
             from .nested import solve_shifts_direct
             x, self.last_residual = solve_shifts_direct(g, JU, b, shifts, adjoint=adjoint, **opts)
+        elif solver == "gmres" and opts.get("precond") == "nested" and \
+                opts.get("groups") == "adaptive":
+            from .gmres import GMRESInfo
+            from .nested import solve_shifts_adaptive
+            opts = dict(opts)
+            opts.pop("precond")
+            opts.pop("groups")
+            x, groups, exact = solve_shifts_adaptive(g, JU, b, shifts, adjoint=adjoint,
+                                                     point=(p, mach, g, X, V, U), **opts)
+            it = np.zeros(S, int)
+            res = np.zeros(S)
+            conv = np.zeros(S, bool)
+            for idx, _, inf in groups:
+                ok = inf.converged
+                it[idx[ok]], res[idx[ok]], conv[idx[ok]] = (inf.iterations[ok],
+                                                            inf.residuals[ok], True)
+            conv[exact] = True
+            self.last_gmres = GMRESInfo(iterations=it, residuals=res, converged=conv,
+                                        restarts=sum(inf.restarts for _, _, inf in groups),
+                                        seconds=sum(inf.seconds for _, _, inf in groups))
+            self.last_groups = groups
+            self.last_exact = exact
         elif solver == "gmres" and opts.get("precond") == "nested":
             from .gmres import GMRESInfo
             from .nested import solve_shifts_anchored

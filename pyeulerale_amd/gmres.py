@@ -58,7 +58,8 @@ class BatchedGMRES:
     def __init__(self, g: K.Grid, JU: torch.Tensor | None, shifts, *, adjoint: bool = False,
                  point=None, precond: str | None = "line", operator: str = "blocks",
                  restart: int = 30, maxiter: int = 600, rtol: float = 1e-10,
-                 graphs: bool = True, check_every: int = 10, factor=None):
+                 graphs: bool = True, check_every: int = 10, factor=None,
+                 give_up: bool = False):
         self.g = g
         self.adjoint = bool(adjoint)
         self.shifts = np.asarray(shifts, dtype=complex).reshape(-1)
@@ -71,6 +72,9 @@ class BatchedGMRES:
         self.rtol = float(rtol)
         self.graphs = bool(graphs)
         self.check_every = max(1, int(check_every))
+        # stop early once every unconverged shift's observed rate of residual reduction, over
+        # the last check_every steps, predicts more than maxiter steps in all
+        self.give_up = bool(give_up)
         dev = JU.device if JU is not None else point[5].device
         self.device = dev
         self.sh = torch.as_tensor(self.shifts, dtype=C128, device=dev).contiguous()
@@ -184,6 +188,18 @@ class BatchedGMRES:
                   _p(st["est"]), K._stream())
         st["hist"][j].copy_(st["est"])
 
+    def _hopeless(self, est, prev, active, total) -> bool:
+        """True when every active shift above rtol, reducing its residual estimate at the rate of
+        the last check_every steps, would need more than maxiter steps in all."""
+        todo = active & (est > self.rtol)
+        if not todo.any():
+            return False
+        with np.errstate(divide="ignore", invalid="ignore"):
+            rate = np.log(np.maximum(est[todo], 1e-300) / np.maximum(prev[todo], 1e-300)) \
+                / self.check_every
+            need = np.where(rate < 0, np.log(self.rtol / est[todo]) / rate, np.inf)
+        return bool((total + need > self.maxiter).all())
+
     def _dots(self, st, V, q, w, out):
         """out[s, i] = sum_k conj(V[i, s, k]) w[s, k] for i < q (basis V of shape (rows, S, n))."""
         n = st["n"]
@@ -247,6 +263,8 @@ class BatchedGMRES:
         history = []
         total = restarts = 0
         graphs = None
+        gave_up = False
+        prev_est = None
         while True:
             # r = b - A x; the true residual decides convergence, also after the last cycle
             self.apply(x, w)
@@ -254,7 +272,7 @@ class BatchedGMRES:
             beta = torch.linalg.vector_norm(r.reshape(S, n), dim=1)
             resid = beta.cpu().numpy() / bnorm
             converged = resid <= self.rtol
-            if converged.all() or total >= self.maxiter:
+            if converged.all() or total >= self.maxiter or gave_up:
                 break
             st["V"][0] = r.reshape(S, n) / beta.clamp_min(1e-300)[:, None]
             for key in ("H", "cs", "sn", "g"):
@@ -276,6 +294,11 @@ class BatchedGMRES:
                     est = (st["est"] / bnorm_d).cpu().numpy()
                     if not (active & (est > self.rtol)).any() or total >= self.maxiter:
                         break
+                    if self.give_up and prev_est is not None and self._hopeless(
+                            est, prev_est, active, total):
+                        gave_up = True
+                        break
+                    prev_est = est
             hist = (st["hist"][:j_done] / bnorm_d).cpu().numpy()
             for jj in range(j_done):
                 iters[active] += 1

@@ -514,3 +514,63 @@ def solve_shifts_anchored(g: K.Grid, JU: torch.Tensor, b: torch.Tensor, shifts, 
         infos.append((idx, anchor, gm.info))
         del gm, f
     return x, infos
+
+
+def solve_shifts_adaptive(g: K.Grid, JU: torch.Tensor, b: torch.Tensor, shifts, *,
+                          adjoint: bool = False, point=None, operator: str = "blocks",
+                          rtol: float = 1e-10, budget: int = 40, ratio: float = 8.0,
+                          shrink: float = 3.0, min_ratio: float = 1.5, check_every: int = 5,
+                          refine: int = 1):
+    """All shifts, grouped as their convergence allows (one factorisation per group or shift).
+
+    Going up in |s|: the next group is every unsolved shift with |s| at most ``ratio`` times the
+    smallest unsolved |s|, anchored at its middle shift (in log |s|).  Batched GMRES with the
+    anchor's exact factorisation as preconditioner solves the group in one multi-right-hand-side
+    elimination per Arnoldi step; it stops once the observed rate of every unconverged shift
+    predicts more than ``budget`` steps (``BatchedGMRES(give_up=True)``), so a group whose shifts
+    sit among lightly damped eigenvalues costs a few steps, not the budget.  Unconverged shifts
+    stay unsolved and start the next group.  When a group converges fewer than half of its
+    shifts besides the anchor (which its own factorisation always solves), the ratio shrinks by
+    ``shrink``; below ``min_ratio`` every remaining shift is solved by its own factorisation (the
+    lightly damped band, where a neighbouring shift's factorisation does not precondition).
+    Returns (x, infos, exact) with infos [(indices, anchor, GMRESInfo)] and exact the indices
+    solved one factorisation each."""
+    from .gmres import BatchedGMRES
+    shifts = np.asarray(shifts, dtype=complex).reshape(-1)
+    x = torch.empty_like(b, dtype=C128)
+    mag = np.abs(shifts)
+    pending = list(np.argsort(mag, kind="stable"))
+    infos, exact = [], []
+    direct = False
+    while pending:
+        if direct:
+            k = pending.pop(0)
+            f = Factorization(g, JU, shifts[k], adjoint=adjoint)
+            x[k] = f.solve(b[k:k + 1], refine=refine)[0]
+            exact.append(int(k))
+            del f
+            continue
+        lo = mag[pending[0]]
+        group = [k for k in pending if mag[k] <= ratio * max(lo, 1e-300)] or pending[:1]
+        anchor_k = group[len(group) // 2]
+        f = Factorization(g, JU, shifts[anchor_k], adjoint=adjoint)
+        idx = np.array(sorted(group))
+        ii = torch.as_tensor(idx, device=b.device)
+        gm = BatchedGMRES(g, JU, shifts[idx], adjoint=adjoint, point=point, precond="nested",
+                          operator=operator, rtol=rtol, restart=budget, maxiter=budget,
+                          factor=f, give_up=True, check_every=check_every)
+        x[ii] = gm.solve(b[ii])
+        conv = gm.info.converged.copy()
+        pos = int(np.flatnonzero(idx == anchor_k)[0])
+        if not conv[pos]:   # the anchor's own system: one exact solve with its factorisation
+            x[anchor_k] = f.solve(b[anchor_k:anchor_k + 1], refine=refine)[0]
+            conv[pos] = True
+            gm.info.converged[pos] = True
+        infos.append((idx, complex(shifts[anchor_k]), gm.info))
+        solved = set(int(k) for k in idx[conv])
+        pending = [k for k in pending if int(k) not in solved]
+        if len(idx) > 1 and conv.sum() - 1 < (len(idx) - 1) / 2:
+            ratio /= shrink
+            direct = ratio < min_ratio
+        del gm, f
+    return x, infos, exact

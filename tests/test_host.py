@@ -234,3 +234,24 @@ def test_face_tile_registry_and_json_cache(tmp_path):
     assert K.face_tile(1, 64, 128) == (2, 32, 8) and K.face_tile(4)[0] == 1
     K.set_face_tile(1, 64, 128, -1)
     K.set_face_tile(4, 0, 0, -1)
+
+
+def test_gmres_gives_up_only_when_every_shift_would_miss_the_budget():
+    """BatchedGMRES(give_up=True) stops a group once every unconverged shift, reducing its
+    residual estimate at the rate of the last check_every steps, would need more than maxiter
+    steps in all; converged and inactive shifts do not count."""
+    from types import SimpleNamespace
+    from pyeulerale_amd.gmres import BatchedGMRES
+    ns = SimpleNamespace(rtol=1e-10, check_every=5, maxiter=40)
+    hopeless = lambda est, prev, active, total: BatchedGMRES._hopeless(
+        ns, np.asarray(est), np.asarray(prev), np.asarray(active), total)
+    on = [True, True]
+    # 10x per 5 steps from 1e-3 at step 10: 35 more steps, 45 > 40
+    assert hopeless([1e-3, 1e-11], [1e-2, 1e-9], on, 10)
+    # 100x per 5 steps: 17.5 more steps, 27.5 <= 40
+    assert not hopeless([1e-3, 1e-11], [1e-1, 1e-9], on, 10)
+    # one shift in reach keeps the group going
+    assert not hopeless([1e-3, 1e-3], [1e-2, 1e-1], on, 10)
+    # stagnation is hopeless; a shift below rtol or inactive is ignored
+    assert hopeless([1e-3, 1e-3], [1e-3, 1e-1], [True, False], 10)
+    assert not hopeless([1e-11, 1e-12], [1e-3, 1e-3], on, 10)
