@@ -114,7 +114,8 @@ PEA_HD void face_flux2_lds(const pea_params& P, const St& st, const T& pLL, cons
 // line, |p_prev - 2 p + p_next| / (p_prev + 2 p + p_next) with prev the lower index.  L's side
 // reads its sensors in the reference's order (f90:209-210) and gets them bitwise; R's side
 // (f90 evaluates sensor(rr, r, l) and sensor(r, l, ll)) gets the same triples summed from the
-// other end, which can differ by one rounding.
+// other end, which can differ by one rounding.  The flux terms are regrouped per face (below),
+// a few roundings from the reference's association.
 template <class St, class OutL, class OutR>
 PEA_HD void face_flux2_cell(const pea_params& P, const St& st, double pL, double pR, double cL,
                             double cR, double iL, double iR, double nuL, double nuR,
@@ -130,22 +131,31 @@ PEA_HD void face_flux2_cell(const pea_params& P, const St& st, double pL, double
         if (dissL) side_coefs(P, lamL, lamR, nuL, nuR, e2L, e4L);
         if (dissR) side_coefs(P, lamR, lamL, nuR, nuL, e2R, e4R);
     }
+    // Per component v the central flux minus the ALE term,
+    //   t_v = (ph_L,v + ph_R,v)/2 - (L_v + R_v) vn/2
+    //       = L_v (qL/2 - vn/2) + R_v (qR/2 - vn/2) + n_v (pL + pR)/2     (v = 1, 2)
+    //                                               + (pL qL + pR qR)/2    (v = 3),
+    // and each side's dissipation e2 d2 - e4 d4 with d4 = (rr - ll) - 3 d2 as
+    // (e2 + 3 e4) d2 - e4 (rr - ll): the face's scalars are formed once, each component then
+    // costs a few FMAs.  f_L = t - D_L and f_R = D_R - t.
     const double hvn = vn / 2.0;
+    const double aL = 0.5 * qL - hvn, aR = 0.5 * qR - hvn;
+    const double hp = 0.5 * (pL + pR), ep = 0.5 * (pL * qL + pR * qR);
+    const double cL2 = e2L + 3.0 * e4L, cR2 = e2R + 3.0 * e4R;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const double Lv = st(1, v), Rv = st(2, v);
-        const double phL = v == 0 ? Lv * qL : v == 3 ? (Lv + pL) * qL : Lv * qL + n[v - 1] * pL;
-        const double phR = v == 0 ? Rv * qR : v == 3 ? (Rv + pR) * qR : Rv * qR + n[v - 1] * pR;
-        const double cen = (phL + phR) / 2.0;
-        const double ale = (Lv + Rv) * hvn;
-        double a = cen, b = -cen;
+        double t = Lv * aL + Rv * aR;
+        if (v == 1 || v == 2) t = t + n[v - 1] * hp;
+        if (v == 3) t = t + ep;
+        double fL = t, fR = -t;
         if (dissL || dissR) {
-            const double d2 = Rv - Lv, d4 = st(3, v) - 3.0 * Rv + 3.0 * Lv - st(0, v);
-            if (dissL) a = a - (e2L * d2 - e4L * d4);
-            if (dissR) b = b + (e2R * d2 - e4R * d4);
+            const double d2 = Rv - Lv, dd = st(3, v) - st(0, v);
+            if (dissL) fL = t - (cL2 * d2 - e4L * dd);
+            if (dissR) fR = (cR2 * d2 - e4R * dd) - t;
         }
-        outL(v, a - ale);
-        outR(v, b + ale);
+        outL(v, fL);
+        outR(v, fR);
     }
 }
 
