@@ -87,24 +87,29 @@ PEA_HD void face_flux2_lds(const pea_params& P, const St& st, const T& pLL, cons
             side_coefs(P, lamR, lamL, sensor(pRR, pR, pL), sensor(pR, pL, pLL), e2R, e4R);
         }
     }
-    const G hvn = vn / 2.0;   // (L + R)/2 vn == (L + R)(vn/2): halving is exact
+    // Central minus ALE and the dissipation regrouped per face as in face_flux2_cell:
+    // t_v = L_v (qL/2 - vn/2) + R_v (qR/2 - vn/2) (+ n_v (pL + pR)/2, or (pL qL + pR qR)/2 for
+    // v = 3), each side's e2 d2 - e4 d4 = (e2 + 3 e4) d2 - e4 (rr - ll), with the second
+    // difference seen from L (R's side sees its negative; the reference sums R's in its own
+    // order, a rounding apart).  f_L = t - D_L, f_R = D_R - t.
+    const G hvn = vn / 2.0;
+    const T aL = 0.5 * qL - hvn, aR = 0.5 * qR - hvn;
+    const T hp = 0.5 * (pL + pR), ep = 0.5 * (pL * qL + pR * qR);
+    const T cL2 = e2L + 3.0 * e4L, cR2 = e2R + 3.0 * e4R;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const T Lv = st(1, v), Rv = st(2, v);
-        const T phL = v == 0 ? Lv * qL : v == 3 ? (Lv + pL) * qL : Lv * qL + n[v - 1] * pL;
-        const T phR = v == 0 ? Rv * qR : v == 3 ? (Rv + pR) * qR : Rv * qR + n[v - 1] * pR;
-        const T cen = (phL + phR) / 2.0;
-        const T ale = (Lv + Rv) * hvn;
-        T a = cen, b = -cen;
+        T t = Lv * aL + Rv * aR;
+        if (v == 1 || v == 2) t = t + n[v - 1] * hp;
+        if (v == 3) t = t + ep;
+        T fL = t, fR = -t;
         if (dissL || dissR) {
-            // second and fourth differences seen from L; R's side sees their negatives (the
-            // reference sums R's in its own order, a rounding apart)
-            const T d2 = Rv - Lv, d4 = st(3, v) - 3.0 * Rv + 3.0 * Lv - st(0, v);
-            if (dissL) a = a - (e2L * d2 - e4L * d4);
-            if (dissR) b = b + (e2R * d2 - e4R * d4);
+            const T d2 = Rv - Lv, dd = st(3, v) - st(0, v);
+            if (dissL) fL = t - (cL2 * d2 - e4L * dd);
+            if (dissR) fR = (cR2 * d2 - e4R * dd) - t;
         }
-        outL(v, a - ale);
-        outR(v, b + ale);
+        outL(v, fL);
+        outR(v, fR);
     }
 }
 
