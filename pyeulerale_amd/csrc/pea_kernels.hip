@@ -1885,6 +1885,42 @@ __device__ __forceinline__ void face_vjp_tile(const pea_params& P, const FsDir& 
         return r;
     };
 
+    // 0. the vertices and velocities of this thread's angular face (step 2) and radial face
+    // (step 3), loaded before the staging so that their latency overlaps it (indices clamped to
+    // the grid for threads without a face).  An angular face across the seam also needs vertex
+    // column 0, R's own.
+    constexpr int NFA = TM * (TN + 3), NFR = (TM + 3) * TN;
+    static_assert(NFA <= NT && NFR <= NT, "one angular and one radial face per thread");
+    const int fa_tm = tid % TM, fa_j = tid / TM;
+    const int fa_m = m0 + fa_tm, fa_Lc = wrap_any(n0 - 2 + fa_j, g.N);
+    const bool fa_act = tid < NFA && fa_m < g.M;
+    const bool fa_seam = fa_Lc + 1 == g.N;
+    T axa[2], axb[2], ava[2], avb[2], sxa[2], sxb[2], sva[2], svb[2];
+    {
+        const int mm = fa_act ? fa_m : 0, cc = fa_act ? fa_Lc + 1 : 0;
+        vtx(mm, cc, axa);
+        vtx(mm + 1, cc, axb);
+        ld.velocity(cc * (g.M + 1) + mm, ava);
+        ld.velocity(cc * (g.M + 1) + mm + 1, avb);
+        if (fa_act && fa_seam) {
+            vtx(mm, 0, sxa);
+            vtx(mm + 1, 0, sxb);
+            ld.velocity(mm, sva);
+            ld.velocity(mm + 1, svb);
+        }
+    }
+    const int fr_i = tid % (TM + 3), fr_tn = tid / (TM + 3);
+    const int fr_n = n0 + fr_tn, fr_Lm = m0 - 2 + fr_i;
+    const bool fr_act = tid < NFR && fr_n < g.N && fr_Lm >= 0 && fr_Lm <= g.M - 2;
+    T rxa[2], rxb[2], rva[2], rvb[2];
+    {
+        const int mm = fr_act ? fr_Lm + 1 : 0, nn = fr_act ? fr_n : 0;
+        vtx(mm, nn + 1, rxa);
+        vtx(mm, nn, rxb);
+        ld.velocity((nn + 1) * (g.M + 1) + mm, rva);
+        ld.velocity(nn * (g.M + 1) + mm, rvb);
+    }
+
     // 1. per-cell pressures, sound speeds, seeds; ghosts; zero accumulators
     for (int i = tid; i < BOX; i += NT) {
         const int bm = i % BM, bn = i / BM;
@@ -1950,13 +1986,9 @@ __device__ __forceinline__ void face_vjp_tile(const pea_params& P, const FsDir& 
 
     // 2. angular faces (m, Lc) | (m, Lc+1) for the tile's rows, Lc from n0-2 to n0+TN
     {
-        constexpr int NF = TM * (TN + 3);
-        for (int base = 0; base < NF; base += NT) {
-            const int f = base + tid;
-            const int tm = f % TM, j = f / TM;
-            const int m = m0 + tm;
-            const bool act = f < NF && m < g.M;
-            const int Lc = wrap_any(n0 - 2 + j, g.N);
+        {
+            const int tm = fa_tm, m = fa_m, Lc = fa_Lc;
+            const bool act = fa_act;
             FaceAdj<T, L> a;
             if (act) {
                 face_adj_zero(a);
@@ -1970,8 +2002,11 @@ __device__ __forceinline__ void face_vjp_tile(const pea_params& P, const FsDir& 
                 box(bLL, LLs);
                 box(bRR, RRs);
                 T nL[2], nRn[2], gv[2];
-                edge(m, Lc + 1, m + 1, Lc + 1, nL, gv);     // L's fi -> fo
-                nrm(m + 1, Rc, m, Rc, nRn);                 // R's bo -> bi
+                edge_normal(axa, axb, nL);                  // L's fi -> fo
+                gv[0] = (ava[0] + avb[0]) / 2.0;
+                gv[1] = (ava[1] + avb[1]) / 2.0;
+                if (fa_seam) edge_normal(sxb, sxa, nRn);    // R's bo -> bi (column 0)
+                else edge_normal(axb, axa, nRn);
                 const T zero[4][L] = {{-0.0, -0.0}, {-0.0, -0.0}, {-0.0, -0.0}, {-0.0, -0.0}};
                 if (Lc + 1 == g.N) {
                     // seam: L's face uses vertex column N, R's face column 0 -- reverse each
@@ -1980,7 +2015,9 @@ __device__ __forceinline__ void face_vjp_tile(const pea_params& P, const FsDir& 
                                             sP[bRR], sC[bL], sC[bR], nL, nRn, gv, true, false,
                                             seed(bL).w, zero, a);
                     T nL0[2], gv0[2];
-                    edge(m, 0, m + 1, 0, nL0, gv0);
+                    edge_normal(sxa, sxb, nL0);
+                    gv0[0] = (sva[0] + svb[0]) / 2.0;
+                    gv0[1] = (sva[1] + svb[1]) / 2.0;
                     face_flux2_rev<T, T, L>(P, LLs, Ls, Rs, RRs, sP[bLL], sP[bL], sP[bR],
                                             sP[bRR], sC[bL], sC[bR], nL0, nRn, gv0, false, true,
                                             zero, seed(bR).w, a);
@@ -2002,12 +2039,9 @@ __device__ __forceinline__ void face_vjp_tile(const pea_params& P, const FsDir& 
     }
     // 3. radial faces (Lm, n) | (Lm+1, n) for the tile's columns, Lm from m0-2 to m0+TM
     {
-        constexpr int NF = (TM + 3) * TN;
-        for (int base = 0; base < NF; base += NT) {
-            const int f = base + tid;
-            const int i = f % (TM + 3), tn = f / (TM + 3);
-            const int n = n0 + tn, Lm = m0 - 2 + i;
-            const bool act = f < NF && n < g.N && Lm >= 0 && Lm <= g.M - 2;
+        {
+            const int tn = fr_tn, n = fr_n, Lm = fr_Lm;
+            const bool act = fr_act;
             FaceAdj<T, L> a;
             if (act) {
                 face_adj_zero(a);
@@ -2018,8 +2052,10 @@ __device__ __forceinline__ void face_vjp_tile(const pea_params& P, const FsDir& 
                 box(bidx(Lm - 1, bn), LLs);    // the wall ghost for Lm = 0
                 box(bidx(Lm + 2, bn), RRs);    // the far-field ghost for Lm + 1 = M - 1
                 T nL[2], nRn[2], gv[2];
-                edge(Lm + 1, n + 1, Lm + 1, n, nL, gv);      // L's fo -> bo
-                nrm(Lm + 1, n, Lm + 1, n + 1, nRn);          // R's bi -> fi
+                edge_normal(rxa, rxb, nL);                   // L's fo -> bo
+                gv[0] = (rva[0] + rvb[0]) / 2.0;
+                gv[1] = (rva[1] + rvb[1]) / 2.0;
+                edge_normal(rxb, rxa, nRn);                  // R's bi -> fi
                 face_flux2_rev<T, T, L>(P, LLs, Ls, Rs, RRs, sP[bidx(Lm - 1, bn)],
                                         sP[bidx(Lm, bn)], sP[bidx(Lm + 1, bn)],
                                         sP[bidx(Lm + 2, bn)], sC[bidx(Lm, bn)],
