@@ -167,32 +167,34 @@ PEA_HD void face_lin_apply(const pea_params& P, const FaceLin& f, const St& st, 
     double e2R = 0.0, e4R = 0.0, de2R = 0.0, de4R = 0.0;
     if (bL & kLinDiss) side_lin_apply(P, f.L, bL, dqL, dqR, dL0, dR0, dp, e2L, e4L, de2L, de4L);
     if (bR & kLinDiss) side_lin_apply(P, f.R, bR, dqL, dqR, dL0, dR0, dp, e2R, e4R, de2R, de4R);
+    // Regrouped as face_flux2_cell: the tangent of central minus ALE,
+    //   t_v = dL_v (qL/2 - vn/2) + dR_v (qR/2 - vn/2) + L_v dqL/2 + R_v dqR/2
+    //         (+ n_v (dpL + dpR)/2 for v = 1, 2; + (dpL qL + dpR qR + pL dqL + pR dqR)/2 for v = 3),
+    // and of each side's e2 d2 - e4 d4 with d4 = (rr - ll) - 3 d2:
+    //   (de2 + 3 de4) d2 - de4 (rr - ll) + (e2 + 3 e4) dd2 - e4 (drr - dll).
+    // f_L = t - D_L, f_R = D_R - t.
+    const double aL = 0.5 * qL - f.hvn, aR = 0.5 * qR - f.hvn;
+    const double hqL = 0.5 * dqL, hqR = 0.5 * dqR;
+    const double hp = 0.5 * (dp[1] + dp[2]);
+    const double ep = 0.5 * (dp[1] * qL + dp[2] * qR) + (pL * hqL + pR * hqR);
+    const double cL2 = e2L + 3.0 * e4L, cR2 = e2R + 3.0 * e4R;
+    const double dcL2 = de2L + 3.0 * de4L, dcR2 = de2R + 3.0 * de4R;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const double Lv = st(1, v), Rv = st(2, v), dLv = dst(1, v), dRv = dst(2, v);
-        double phL, phR;
-        if (v == 0) {
-            phL = dLv * qL + Lv * dqL;
-            phR = dRv * qR + Rv * dqR;
-        } else if (v == 3) {
-            phL = (dLv + dp[1]) * qL + (Lv + pL) * dqL;
-            phR = (dRv + dp[2]) * qR + (Rv + pR) * dqR;
-        } else {
-            const double nv = v == 1 ? f.n0 : f.n1;
-            phL = dLv * qL + Lv * dqL + nv * dp[1];
-            phR = dRv * qR + Rv * dqR + nv * dp[2];
-        }
-        const double cen = (phL + phR) / 2.0;
-        const double ale = (dLv + dRv) * f.hvn;
-        double a = cen, b = -cen;
+        double t = dLv * aL + dRv * aR + Lv * hqL + Rv * hqR;
+        if (v == 1) t = t + f.n0 * hp;
+        if (v == 2) t = t + f.n1 * hp;
+        if (v == 3) t = t + ep;
+        double fL = t, fR = -t;
         if ((bL | bR) & kLinDiss) {
-            const double d2 = Rv - Lv, d4 = st(3, v) - 3.0 * Rv + 3.0 * Lv - st(0, v);
-            const double dd2 = dRv - dLv, dd4 = dst(3, v) - 3.0 * dRv + 3.0 * dLv - dst(0, v);
-            if (bL & kLinDiss) a = a - (de2L * d2 + e2L * dd2 - (de4L * d4 + e4L * dd4));
-            if (bR & kLinDiss) b = b + (de2R * d2 + e2R * dd2 - (de4R * d4 + e4R * dd4));
+            const double d2 = Rv - Lv, dd = st(3, v) - st(0, v);
+            const double dd2 = dRv - dLv, ddd = dst(3, v) - dst(0, v);
+            if (bL & kLinDiss) fL = t - (dcL2 * d2 - de4L * dd + (cL2 * dd2 - e4L * ddd));
+            if (bR & kLinDiss) fR = (dcR2 * d2 - de4R * dd + (cR2 * dd2 - e4R * ddd)) - t;
         }
-        outL(v, a - ale);
-        outR(v, b + ale);
+        outL(v, fL);
+        outR(v, fR);
     }
 }
 
