@@ -168,12 +168,12 @@ PEA_HD void face_flux2_cell(const pea_params& P, const St& st, double pL, double
 // Reverse of side_dissipation with seed db on D: accumulates into the adjoints of the side's
 // ll, l, r, rr states (sb*), of its two wave integrals (lamb_l, lamb_r) and of the four
 // pressures entering its sensors (pb_ll, pb_l, pb_r, pb_rr).  Switch decisions are recomputed.
-// d2, d4: the face's second and fourth state differences seen from its L side (r - l and
-// rr - 3r + 3l - ll); flip = true for the R side, which sees their negatives.
+// d2, dd: the face's differences r - l and rr - ll seen from its L side (the fourth difference
+// is dd - 3 d2); flip = true for the R side, which sees their negatives.
 template <class T, int L>
 PEA_HD void side_dissipation_rev(const pea_params& P, const T& lam_l, const T& lam_r,
                                  const T& pll, const T& pl, const T& pr, const T& prr,
-                                 const T* d2, const T* d4, bool flip,
+                                 const T* d2, const T* dd, bool flip,
                                  const T (*db)[L], T (*sbll)[L], T (*sbl)[L], T (*sbr)[L],
                                  T (*sbrr)[L], T* lamb_l, T* lamb_r, T* pbll, T* pbl, T* pbr,
                                  T* pbrr) {
@@ -190,20 +190,23 @@ PEA_HD void side_dissipation_rev(const pea_params& P, const T& lam_l, const T& l
     const T e4 = zero4 ? lit<T>(0.0) : t4;
     const double sg1 = sign1(a1), sg2 = sign1(a2);
     const T ib1 = recip(b1), ib2 = recip(b2);
+    // D = e2 d2 - e4 d4 with d4 = dd - 3 d2: the states see (e2 + 3 e4) d2 - e4 dd
+    const T c2 = e2 + 3.0 * e4;
 #pragma unroll
     for (int k = 0; k < L; ++k) {
-        T e2b = adj_zero<T>(), e4b = adj_zero<T>();
+        T e2b = adj_zero<T>(), sdd = adj_zero<T>();
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const T d = db[v][k], x = flip ? -d : d;
             e2b = e2b + x * d2[v];
-            e4b = e4b - x * d4[v];
-            const T t2 = d * e2, tq = d * e4;
-            sbr[v][k] = sbr[v][k] + t2 + 3.0 * tq;
-            sbl[v][k] = sbl[v][k] - t2 - 3.0 * tq;
+            sdd = sdd + x * dd[v];
+            const T t2 = d * c2, tq = d * e4;
+            sbr[v][k] = sbr[v][k] + t2;
+            sbl[v][k] = sbl[v][k] - t2;
             sbrr[v][k] = sbrr[v][k] - tq;
             sbll[v][k] = sbll[v][k] + tq;
         }
+        T e4b = 3.0 * e2b - sdd;   // -(sum of x d4)
         const T t4b = zero4 ? lit<T>(0.0) : e4b;
         T lamb = P.k4 * t4b;
         e2b = e2b - P.c4 * t4b;
@@ -279,11 +282,11 @@ PEA_HD void face_flux2_rev(const pea_params& P, const T* LL, const T* Lc, const 
     }
     if (dissL || dissR) {
         const G sn = ssqrt(n[0] * n[0] + n[1] * n[1]);
-        T d2[4], d4[4];
+        T d2[4], dd[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             d2[v] = R[v] - Lc[v];
-            d4[v] = RR[v] - 3.0 * R[v] + 3.0 * Lc[v] - LL[v];
+            dd[v] = RR[v] - LL[v];
         }
         if (dissL) {
             const double sL = sign1(qL), sR = sign1(qR);
@@ -295,7 +298,7 @@ PEA_HD void face_flux2_rev(const pea_params& P, const T* LL, const T* Lc, const 
 #pragma unroll
                 for (int v = 0; v < 4; ++v) db[v][k] = -fbL[v][k];
             }
-            side_dissipation_rev<T, L>(P, lamL, lamR, pLL, pL, pR, pRR, d2, d4, false, db,
+            side_dissipation_rev<T, L>(P, lamL, lamR, pLL, pL, pR, pRR, d2, dd, false, db,
                                        a.u[0], a.u[1], a.u[2], a.u[3], lbl, lbr, a.p[0],
                                        a.p[1], a.p[2], a.p[3]);
 #pragma unroll
@@ -318,7 +321,7 @@ PEA_HD void face_flux2_rev(const pea_params& P, const T* LL, const T* Lc, const 
 #pragma unroll
                 for (int v = 0; v < 4; ++v) db[v][k] = -fbR[v][k];
             }
-            side_dissipation_rev<T, L>(P, lamR, lamL, pRR, pR, pL, pLL, d2, d4, true, db,
+            side_dissipation_rev<T, L>(P, lamR, lamL, pRR, pR, pL, pLL, d2, dd, true, db,
                                        a.u[3], a.u[2], a.u[1], a.u[0], lbr, lbl, a.p[3],
                                        a.p[2], a.p[1], a.p[0]);
 #pragma unroll
