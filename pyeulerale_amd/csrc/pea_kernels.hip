@@ -145,6 +145,18 @@ __device__ __forceinline__ double buf_load(__amdgpu_buffer_rsrc_t r, int elem, i
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, elem * 8,
                                                                           plane_elems * 8, 0));
 }
+// elements elem and elem + 1 of a plane (one 16-byte load at an 8-byte aligned offset)
+__device__ __forceinline__ double2 buf_load_pair(__amdgpu_buffer_rsrc_t r, int elem,
+                                                 int plane_elems) {
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, elem * 8,
+                                                                            plane_elems * 8, 0));
+}
+// 2-vectors p and p + 1 of a vertex array (two planes of P doubles): one 16-byte load per plane
+__device__ __forceinline__ void vec2_pair(const double* A, int P, int p, double* a, double* b) {
+    const auto r = buf_rsrc(A, 16L * P);
+    const double2 d0 = buf_load_pair(r, p, 0), d1 = buf_load_pair(r, p, P);
+    a[0] = d0.x, b[0] = d0.y, a[1] = d1.x, b[1] = d1.y;
+}
 // complex element (re, im) of an interleaved complex array
 __device__ __forceinline__ double2 buf_load_cx(__amdgpu_buffer_rsrc_t r, int elem, int plane_elems) {
     return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, elem * 16,
@@ -169,6 +181,11 @@ struct LdReal {
         const auto r = buf_rsrc(V, 16L * g.P);
         w[0] = buf_load(r, p, 0);
         w[1] = buf_load(r, p, g.P);
+    }
+    // vertices (velocities) p and p + 1, which are neighbours along m in one vertex column
+    __device__ void vertex2(int p, double* xa, double* xb) const { vec2_pair(X, g.P, p, xa, xb); }
+    __device__ void velocity2(int p, double* wa, double* wb) const {
+        vec2_pair(V, g.P, p, wa, wb);
     }
     PEA_HD double machv() const { return mach; }
     // item b of a batch launch (grid z): the b-th of consecutive state fields
@@ -251,6 +268,11 @@ struct LdDualU {
         w[0] = buf_load(r, p, 0);
         w[1] = buf_load(r, p, g.P);
     }
+    // vertices (velocities) p and p + 1, which are neighbours along m in one vertex column
+    __device__ void vertex2(int p, double* xa, double* xb) const { vec2_pair(X, g.P, p, xa, xb); }
+    __device__ void velocity2(int p, double* wa, double* wb) const {
+        vec2_pair(V, g.P, p, wa, wb);
+    }
     PEA_HD double machv() const { return mach; }
     // item b of a batch launch (grid z): the b-th group of K/2 consecutive complex directions
     PEA_HD LdDualU at(int b) const {
@@ -285,6 +307,11 @@ struct LdCxU {
         const auto r = buf_rsrc(V, 16L * g.P);
         w[0] = buf_load(r, p, 0);
         w[1] = buf_load(r, p, g.P);
+    }
+    // vertices (velocities) p and p + 1, which are neighbours along m in one vertex column
+    __device__ void vertex2(int p, double* xa, double* xb) const { vec2_pair(X, g.P, p, xa, xb); }
+    __device__ void velocity2(int p, double* wa, double* wb) const {
+        vec2_pair(V, g.P, p, wa, wb);
     }
     PEA_HD double machv() const { return mach; }
     // direction b of a batch launch (grid y)
@@ -521,13 +548,23 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const FsDir& 
     G gx[4][2], gw[3][2];   // vertices BI, FI, FO, BO; velocities at FI, FO, BO
     G ex[2][2], ew[2][2];   // tile-edge face: the edge from vertex ex[0] to vertex ex[1]
     auto load_cell = [&](int mc, int nc) {
-        vtx(mc, nc, gx[BI]);
-        vtx(mc, nc + 1, gx[FI]);
-        vtx(mc + 1, nc + 1, gx[FO]);
-        vtx(mc + 1, nc, gx[BO]);
-        vel(mc, nc + 1, gw[0]);
-        vel(mc + 1, nc + 1, gw[1]);
-        vel(mc + 1, nc, gw[2]);
+        if constexpr (kEarlyCell) {
+            // bi, bo and fi, fo are neighbours along m: one 16-byte load per pair and plane
+            const int p = nc * (g.M + 1) + mc, q = p + g.M + 1;
+            G wbi[2];
+            l.vertex2(p, gx[BI], gx[BO]);
+            l.vertex2(q, gx[FI], gx[FO]);
+            l.velocity2(q, gw[0], gw[1]);
+            l.velocity2(p, wbi, gw[2]);
+        } else {
+            vtx(mc, nc, gx[BI]);
+            vtx(mc, nc + 1, gx[FI]);
+            vtx(mc + 1, nc + 1, gx[FO]);
+            vtx(mc + 1, nc, gx[BO]);
+            vel(mc, nc + 1, gw[0]);
+            vel(mc + 1, nc + 1, gw[1]);
+            vel(mc + 1, nc, gw[2]);
+        }
     };
     // the tile-edge face's vertices: threads below TM, the angular face n0-1 | n0 of row
     // m0 + tid (edge bi -> bo); threads TM..TM+TN-1, the wall face of column n0 + tid - TM (edge
@@ -854,14 +891,14 @@ __device__ __forceinline__ void face_dirs_tile(const pea_params& P, const FsDir&
     // geometry of this cell's faces and area, and of the tile-edge face of threads below NE
     double gx[4][2], gw[3][2], ex[2][2], ew[2][2];
     {
+        // bi, bo and fi, fo are neighbours along m: one 16-byte load per pair and plane
         const int mc = m < g.M ? m : g.M - 1, nc = n < g.N ? n : g.N - 1;
-        vtx(mc, nc, gx[BI]);
-        vtx(mc, nc + 1, gx[FI]);
-        vtx(mc + 1, nc + 1, gx[FO]);
-        vtx(mc + 1, nc, gx[BO]);
-        vel(mc, nc + 1, gw[0]);
-        vel(mc + 1, nc + 1, gw[1]);
-        vel(mc + 1, nc, gw[2]);
+        const int p = nc * (g.M + 1) + mc, q = p + g.M + 1;
+        double wbi[2];
+        l.vertex2(p, gx[BI], gx[BO]);
+        l.vertex2(q, gx[FI], gx[FO]);
+        l.velocity2(q, gw[0], gw[1]);
+        l.velocity2(p, wbi, gw[2]);
     }
     if (tid < NE) {
         const int mm = m0 + tid < g.M ? m0 + tid : g.M - 1;
@@ -1897,16 +1934,13 @@ __device__ __forceinline__ void face_vjp_tile(const pea_params& P, const FsDir& 
     const bool fa_seam = fa_Lc + 1 == g.N;
     T axa[2], axb[2], ava[2], avb[2], sxa[2], sxb[2], sva[2], svb[2];
     {
+        // the face's two vertices are neighbours along m: one 16-byte load per plane
         const int mm = fa_act ? fa_m : 0, cc = fa_act ? fa_Lc + 1 : 0;
-        vtx(mm, cc, axa);
-        vtx(mm + 1, cc, axb);
-        ld.velocity(cc * (g.M + 1) + mm, ava);
-        ld.velocity(cc * (g.M + 1) + mm + 1, avb);
+        ld.vertex2(cc * (g.M + 1) + mm, axa, axb);
+        ld.velocity2(cc * (g.M + 1) + mm, ava, avb);
         if (fa_act && fa_seam) {
-            vtx(mm, 0, sxa);
-            vtx(mm + 1, 0, sxb);
-            ld.velocity(mm, sva);
-            ld.velocity(mm + 1, svb);
+            ld.vertex2(mm, sxa, sxb);
+            ld.velocity2(mm, sva, svb);
         }
     }
     const int fr_i = tid % (TM + 3), fr_tn = tid / (TM + 3);
@@ -1934,10 +1968,8 @@ __device__ __forceinline__ void face_vjp_tile(const pea_params& P, const FsDir& 
 #pragma unroll
             for (int v = 0; v < 4; ++v) sU[v][i] = u[v];
             T x[4][2];
-            vtx(cm, cn, x[BI]);
-            vtx(cm, cn + 1, x[FI]);
-            vtx(cm + 1, cn + 1, x[FO]);
-            vtx(cm + 1, cn, x[BO]);
+            ld.vertex2(cn * (g.M + 1) + cm, x[BI], x[BO]);
+            ld.vertex2((cn + 1) * (g.M + 1) + cm, x[FI], x[FO]);
             const T d1x = x[FO][0] - x[BI][0], d1z = x[FO][1] - x[BI][1];
             const T d2x = x[BO][0] - x[FI][0], d2z = x[BO][1] - x[FI][1];
             const T ia = rcp_fast(sabs(d1x * d2z - d1z * d2x) / 2.0);
