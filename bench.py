@@ -69,8 +69,8 @@ def parse():
     ap.add_argument("--profile", action="store_true",
                     help="plain launches only (no graphs) for counter collection: a 512 MiB "
                          "copy for calibration, the batched residual, the JVP / VJP / "
-                         "complex-step products at 1025x513 and the GMRES line solve, 5 "
-                         "launches each")
+                         "complex-step products at 1025x513 and the GMRES line solve, each "
+                         "repeated for about 60 ms")
     ap.add_argument("--only-gmres", action="store_true",
                     help="only the 1025x513 batched GMRES section (its JSON on one line), for "
                          "a kernel trace of the Arnoldi steps")
@@ -131,15 +131,16 @@ def graph_time_ms(torch, fn, reps=20):
 PMC_SUMMARY = ROOT / "profiles" / "pmc_face_kernels_1025x513.md"
 
 
-def pmc_figures() -> dict:
-    """HBM traffic and FP64 flops per launch of the batched residual kernel, as measured by the
-    rocprofv3 counter passes summarised in profiles/ (FETCH_SIZE and WRITE_SIZE calibrated
-    against a 512 MiB copy; SQ_INSTS_VALU_*_F64).  Empty when the summary is absent."""
+def pmc_figures(tm: int, tn: int) -> dict:
+    """HBM traffic and FP64 flops per launch of the 8-state 1025x513 residual with tm x tn tiles,
+    as measured by the rocprofv3 counter passes summarised in profiles/ (FETCH_SIZE and
+    WRITE_SIZE calibrated against a 512 MiB copy; SQ_INSTS_VALU_*_F64).  Empty when the summary
+    or the shape's line is absent."""
     out = {}
     try:
         for line in PMC_SUMMARY.read_text().splitlines():
             for key in ("traffic_bytes_per_launch", "fp64_flop_per_launch"):
-                if line.startswith(key + " k_face_fwd batched residual 1025x513"):
+                if line.startswith(f"{key} k_face_fwd batched residual 1025x513 x8 tile {tm}x{tn}:"):
                     out[key] = float(line.rsplit(":", 1)[1])
     except OSError:
         pass
@@ -326,56 +327,65 @@ def residual_beyond_cache(torch, K, p, mach, g, Xd, Vd, Ud, Rd, alpha, alpha_dot
 
 def profile_launches(torch, K):
     """Kernel launches for hardware-counter passes (rocprofv3 --pmc): every dispatch is a plain
-    launch on torch's current stream.  A device-to-device copy of 512 MiB comes first; its
+    launch on torch's current stream.  Device-to-device copies of 512 MiB come first; their
     FETCH_SIZE / WRITE_SIZE give the calibration of the byte counters.  Then the face kernels
     (batched residual at 257x129 and at 1025x513 with 8 states; residual, JVP, VJP and the
     complex step at 1025x513; the tangent and adjoint of 32 directions per launch, and the
     tangent with the primal part once per tile) and the radial-line preconditioner solve of
-    batched GMRES (32 shifts, 1025x513)."""
+    batched GMRES (32 shifts, 1025x513).  The 8-state 1025x513 residual runs with every tile
+    shape of the residual, the others with the default shape.  The inputs are built first; each
+    group of launches is repeated for about 60 ms before its last 5 (3 for the 32-direction
+    launches, 1 for the block assembly), so that the GPU's clock, which falls while the GPU
+    idles and rises over the first ~30 ms of load, has risen; the counts per launch do not
+    depend on the clock."""
     src = torch.empty(64 * 2 ** 20, dtype=torch.float64, device="cuda").fill_(1.0)
     dst = torch.empty_like(src)
     for _ in range(3):
         dst.copy_(src)
     del src, dst
-    mach, aoa = 0.6, 0.0
-    amp = np.deg2rad(0.5)
-    alpha, alpha_dot = amp * np.sin(0.4), amp * 0.2 * np.cos(0.4)
-    X, V, U, M, N = make_case(257, 129, 0.04, mach, aoa, alpha, alpha_dot, 20.0, BATCH)
-    g, p = K.Grid(M, N), K.Params(aoa_deg=aoa)
+
     dev = lambda a: torch.as_tensor(np.ascontiguousarray(np.swapaxes(a, -1, -2))).cuda()
-    Xd, Vd, Ud = dev(X), dev(V), dev(U)
-    Rd = torch.empty_like(Ud)
-    for _ in range(5):
-        K.compute_odes_batch(p, mach, g, Xd, Vd, Ud, Rd)
+    mach2, amp = 0.6, np.deg2rad(0.5)
+    alpha, alpha_dot = amp * np.sin(0.4), amp * 0.2 * np.cos(0.4)
+    X, V, U, M2, N2 = make_case(257, 129, 0.04, mach2, 0.0, alpha, alpha_dot, 20.0, BATCH)
+    g2, p2 = K.Grid(M2, N2), K.Params(aoa_deg=0.0)
+    Xd2, Vd2, Ud2 = dev(X), dev(V), dev(U)
+    Rd2 = torch.empty_like(Ud2)
     X, V, U, M, N = make_case(1025, 513, 0.04, 0.5, 0.0, 0.0, 0.0, 20.0, HEAD_BATCH)
     g, p = K.Grid(M, N), K.Params(aoa_deg=0.0)
-    Xd, Vd, Ud = dev(X), dev(V), dev(U)
-    Rd = torch.empty_like(Ud)
-    for _ in range(5):
-        K.compute_odes_batch(p, 0.5, g, Xd, Vd, Ud, Rd)
-    del Rd
-    Ud = Ud[0].contiguous()
+    Xd, Vd, Ud8 = dev(X), dev(V), dev(U)
+    Rd = torch.empty_like(Ud8)
+    Ud = Ud8[0].contiguous()
     rng = np.random.default_rng(1)
     cshape = (4, M, N)
     dU = dev(rng.standard_normal(cshape) + 1j * rng.standard_normal(cshape))
     w = dev(rng.standard_normal(cshape) + 1j * rng.standard_normal(cshape))
     out = torch.empty_like(dU)
     R1 = torch.empty(g.cell_shape, dtype=torch.float64, device="cuda")
-    for _ in range(5):
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    d32 = torch.randn((32, *g.cell_shape), dtype=torch.complex128, device="cuda", generator=gen)
+    o32 = torch.empty_like(d32)
+    torch.cuda.synchronize()
+
+    for _ in range(1000):
+        K.compute_odes_batch(p2, mach2, g2, Xd2, Vd2, Ud2, Rd2)
+    for t in range(K._lib.load().pea_num_face_tiles()):   # every residual tile shape
+        K.set_face_tile(0, g.M, g.N, t)
+        for _ in range(500):
+            K.compute_odes_batch(p, 0.5, g, Xd, Vd, Ud8, Rd)
+    K.set_face_tile(0, g.M, g.N, -1)
+    for _ in range(150):
         K.compute_odes(p, 0.5, g, Xd, Vd, Ud, R1)
         K.jvp_odes(p, 0.5, g, Xd, Vd, Ud, dU=dU, dR=out)
         K.vjp_odes(p, 0.5, g, Xd, Vd, Ud, w)
         K.cs_jvp_odes(p, 0.5, g, Xd, Vd, Ud, dU=dU, dR=out)
     # 32 directions per launch: one per grid z (Dual tangent, face adjoint), and all of them per
     # workgroup with the primal part once per tile
-    gen = torch.Generator(device="cuda").manual_seed(3)
-    d32 = torch.randn((32, *g.cell_shape), dtype=torch.complex128, device="cuda", generator=gen)
-    o32 = torch.empty_like(d32)
-    for _ in range(3):
+    for _ in range(12):
         K.jvp_odes_multi(p, 0.5, g, Xd, Vd, Ud, dU=d32, dR=o32)
         K.vjp_odes_states_multi(p, 0.5, g, Xd, Vd, Ud, d32, gU=o32)
         K.jvp_odes_dirs(p, 0.5, g, Xd, Vd, Ud, d32, dR=o32)
-    del d32, o32
+    del d32, o32, Rd, Ud8, Rd2, Ud2
     # radial-line preconditioner of batched GMRES: factorisation and solves for 32 shifts
     from pyeulerale_amd.gmres import BatchedGMRES
     JU = K.cs_linearize_odes(p, 0.5, g, Xd, Vd, Ud, want=("states",))[0]
@@ -383,7 +393,8 @@ def profile_launches(torch, K):
     gen = torch.Generator(device="cuda").manual_seed(0)
     rr = torch.randn((32, *g.cell_shape), dtype=torch.complex128, device="cuda", generator=gen)
     zz = torch.empty_like(rr)
-    for _ in range(5):
+    torch.cuda.synchronize()
+    for _ in range(15):
         gm.psolve(rr, zz)
     torch.cuda.synchronize()
 
@@ -784,21 +795,20 @@ def main():
     Ud = torch.as_tensor(np.ascontiguousarray(np.swapaxes(U, -1, -2))).cuda()
     Rd = torch.empty_like(Ud)
     step = lambda: K.compute_odes_batch(p, mach, g, Xd, Vd, Ud, Rd)
-    # The project's autotuner times every residual tile shape on this grid and batch; its
-    # timings are reported.  The timed launches use the default 16x16 tile, which the tuner finds
-    # within noise of the fastest shape here (<= 2%, and the fastest shape changes from run to
-    # run), so that the counter passes of profiles/ (plain launches, default tile) describe the
-    # kernel timed here.
-    tuned_head = K.autotune_face(p, mach, g, Xd, Vd, Ud[0].contiguous(), kinds=("residual",),
-                                 reps=5, batch=HEAD_BATCH)["residual"]
-    K.set_face_tile(0, g.M, g.N, -1)
-    head_tile = K.face_tile(0, g.M, g.N)
     # The K timed launches are replayed from one HIP graph, and so are the warm-up launches after
     # the first, which runs eagerly so that nothing is loaded during capture.  The events bracket
     # the timed replay on the stream the kernels run on.  The GPU's clock rises over the first
-    # ~30 ms of sustained load (a launch takes ~140 us cold, ~119 us once the clock has risen), so
-    # the default warm-up is 1000 launches (~0.12 s) and the steady rate is what is timed.
-    K._lib.load()
+    # ~30 ms of sustained load (a launch takes ~140 us cold, ~115 us once the clock has risen), so
+    # the default warm-up is 1000 launches (~0.12 s) and the steady rate is what is timed.  The
+    # graphs launch the default 16x16 tile.
+    # First the project's autotuner times every residual tile shape on this grid and batch (300
+    # untimed launches of each shape before its 5 timed ones, so that the clock has risen) and
+    # registers the fastest, which the timed launches use.  The counter summary in profiles/
+    # holds every shape of this launch.
+    tuned_head = K.autotune_face(p, mach, g, Xd, Vd, Ud[0].contiguous(), kinds=("residual",),
+                                 reps=5, batch=HEAD_BATCH, warmup=300)["residual"]
+    head_tile = K.face_tile(0, g.M, g.N)
+    assert head_tile[0] == tuned_head[0]
     if args.warmup > 0:
         step()
     graphs = []
@@ -871,7 +881,7 @@ def main():
     if not args.no_c5_freq:
         extra["frequency_response_1025x513"] = freq_response_c5(torch, K)
         progress("1025x513 frequency response done")
-    pmc = pmc_figures()
+    pmc = pmc_figures(head_tile[1], head_tile[2])
     line = {
         "metric": "residual evals/s + hand-written VJP vs complex-step JVP speedup (FP64, 1 GPU)",
         "value": value, "unit": "residual evals/s", "n_gpus": 1, "steps": args.steps,
@@ -890,7 +900,8 @@ def main():
                      "tile_autotuned_fastest": tuned_head[0],
                      "kernel_ms": kern_ms, "algorithmic_bytes_per_launch": bytes_launch,
                      "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE of the 8-state "
-                                       "1025x513 launch, profiles/pmc_face_kernels_1025x513.md",
+                                       "1025x513 launch with this tile shape, "
+                                       "profiles/pmc_face_kernels_1025x513.md",
                      "fp64_tflops": (pmc["fp64_flop_per_launch"] / (kern_ms * 1e-3) / 1e12
                                      if "fp64_flop_per_launch" in pmc else None),
                      "fp64_peak_tflops": FP64_PEAK_TFLOPS},

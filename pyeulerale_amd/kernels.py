@@ -370,8 +370,9 @@ def set_face_tile(kind: int, M: int, N: int, tile: int) -> None:
         _FACE_TUNED[(kind, M, N)] = tile
 
 
-def _time_events(fn, reps: int) -> float:
-    fn()
+def _time_events(fn, reps: int, warmup: int = 1) -> float:
+    for _ in range(max(1, warmup)):
+        fn()
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     a.record()
     for _ in range(reps):
@@ -382,12 +383,13 @@ def _time_events(fn, reps: int) -> float:
 
 
 def autotune_face(p: Params, mach: float, g: Grid, X, V, U, kinds=("residual", "jvp", "vjp"),
-                  reps: int = 10, batch: int = 1) -> dict:
+                  reps: int = 10, batch: int = 1, warmup: int = 1) -> dict:
     """Times every tile shape of the face kernels (residual, states tangent, states adjoint) on
     this grid, ``batch`` states / directions / seeds per launch, and registers the fastest for
-    grids of this size (``pea_set_face_tile``).  The choices are cached per (kind, M, N) for the
-    process; :func:`save_tuning` / :func:`load_tuning` keep them across processes.  Returns
-    {kind: (tile, {tile: ms})}."""
+    grids of this size (``pea_set_face_tile``).  Each shape runs ``warmup`` untimed launches
+    before its ``reps`` timed ones (the GPU's clock rises over the first tens of milliseconds of
+    load).  The choices are cached per (kind, M, N) for the process; :func:`save_tuning` /
+    :func:`load_tuning` keep them across processes.  Returns {kind: (tile, {tile: ms})}."""
     lib = _lib.load()
     gen = torch.Generator(device=U.device).manual_seed(0)
     Ub = U.expand(batch, *U.shape).contiguous()
@@ -406,7 +408,7 @@ def autotune_face(p: Params, mach: float, g: Grid, X, V, U, kinds=("residual", "
         for t in range(lib.pea_num_face_tiles()):
             if lib.pea_set_face_tile(kind, g.M, g.N, t) != 0:
                 continue
-            times[t] = _time_events(runs[name], reps)
+            times[t] = _time_events(runs[name], reps, warmup)
         best = min(times, key=times.get)
         set_face_tile(kind, g.M, g.N, best)
         result[name] = (best, times)
