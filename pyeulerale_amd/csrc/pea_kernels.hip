@@ -1955,33 +1955,52 @@ __device__ __forceinline__ void face_vjp_tile(const pea_params& P, const FsDir& 
         ld.velocity(nn * (g.M + 1) + mm, rvb);
     }
 
-    // 1. per-cell pressures, sound speeds, seeds; ghosts; zero accumulators
-    for (int i = tid; i < BOX; i += NT) {
+    // 1. per-cell pressures, sound speeds, seeds; ghosts; zero accumulators.  Every pass's
+    // loads (the box cell's state, area vertices and seed, the row clamped to the grid) are
+    // issued before the first is used: one round trip instead of one per pass (passes past the
+    // box load a valid cell and discard it).
+    constexpr int NPASS = (BOX + NT - 1) / NT;
+    T su[NPASS][4], sx[NPASS][4][2], sw[NPASS][4][L];
+#pragma unroll
+    for (int k = 0; k < NPASS; ++k) {
+        const int i = tid + k * NT < BOX ? tid + k * NT : BOX - 1;
+        const int bm = i % BM, bn = i / BM;
+        const int cm = m0 - 3 + bm, cmc = cm < 0 ? 0 : (cm >= g.M ? g.M - 1 : cm);
+        const int cn = wrap_any(n0 - 3 + bn, g.N);
+        const int c = cn * g.M + cmc;
+        ld.state(c, su[k]);
+        ld.vertex2(cn * (g.M + 1) + cmc, sx[k][BI], sx[k][BO]);
+        ld.vertex2((cn + 1) * (g.M + 1) + cmc, sx[k][FI], sx[k][FO]);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            sw[k][v][0] = wseed[2 * ((long)v * g.C + c)];
+            sw[k][v][1] = wseed[2 * ((long)v * g.C + c) + 1];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NPASS; ++k) {
+        const int i = tid + k * NT;
+        if (i >= BOX) continue;
         const int bm = i % BM, bn = i / BM;
         const int cm = m0 - 3 + bm;
         const int cn = wrap_any(n0 - 3 + bn, g.N);
         if (cm >= 0 && cm < g.M) {
-            T u[4];
-            const int c = cn * g.M + cm;
-            ld.state(c, u);
+            const T* u = su[k];
             cell_pc(u, sP[i], sC[i]);
 #pragma unroll
             for (int v = 0; v < 4; ++v) sU[v][i] = u[v];
-            T x[4][2];
-            ld.vertex2(cn * (g.M + 1) + cm, x[BI], x[BO]);
-            ld.vertex2((cn + 1) * (g.M + 1) + cm, x[FI], x[FO]);
+            const T(&x)[4][2] = sx[k];
             const T d1x = x[FO][0] - x[BI][0], d1z = x[FO][1] - x[BI][1];
             const T d2x = x[BO][0] - x[FI][0], d2z = x[BO][1] - x[FI][1];
             const T ia = rcp_fast(sabs(d1x * d2z - d1z * d2x) / 2.0);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                sFB[v][0][i] = -(wseed[2 * ((long)v * g.C + c)] * ia);
-                sFB[v][1][i] = -(wseed[2 * ((long)v * g.C + c) + 1] * ia);
+                sFB[v][0][i] = -(sw[k][v][0] * ia);
+                sFB[v][1][i] = -(sw[k][v][1] * ia);
             }
         } else if (cm == -1 || cm == g.M) {
-            const int mc = cm < 0 ? 0 : g.M - 1;
-            T u[4], gh[4];
-            ld.state(cn * g.M + mc, u);
+            const T* u = su[k];   // the state of row 0 or M - 1 (the row clamped to the grid)
+            T gh[4];
             if (cm < 0) {
                 T nr[2], gv[2];
                 edge(0, cn, 0, cn + 1, nr, gv);
