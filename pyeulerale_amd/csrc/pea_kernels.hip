@@ -624,10 +624,9 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const FsDir& 
             sP[i] = pressure_q(gh);
         }
     };
-    // FP64 states: every pass's loads are issued before the first is used (one round trip
-    // instead of one per pass; passes past the box load a valid cell and discard it).  States
-    // with tangents would hold a pass's 4 x W doubles more; they are loaded pass by pass.
-    if constexpr (std::is_same<T, double>::value) {
+    // Every pass's loads are issued before the first is used (one round trip instead of one per
+    // pass; passes past the box load a valid cell and discard it).
+    {
         constexpr int NPASS = (BOX + NT - 1) / NT;
         T u[NPASS][4];
 #pragma unroll
@@ -636,12 +635,6 @@ __device__ __forceinline__ void face_fwd_tile(const pea_params& P, const FsDir& 
 #pragma unroll
         for (int k = 0; k < NPASS; ++k)
             if (tid + k * NT < BOX) stage_cell(tid + k * NT, u[k]);
-    } else {
-        for (int i = tid; i < BOX; i += NT) {
-            T u[4];
-            stage_load(i, u);
-            stage_cell(i, u);
-        }
     }
     __syncthreads();
     if constexpr (kCell) {
