@@ -1065,13 +1065,17 @@ __device__ __forceinline__ void face_dirs_tile(const pea_params& P, const FsDir&
     const int cme = n * g.M + m;
     const int cw = tn % CPW;
     for (int d = d0; d < d1; ++d) {
+        // every pass's loads are issued before the first is used
+        double2 zz[NPASS][4];
+#pragma unroll
+        for (int k = 0; k < NPASS; ++k)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) zz[k][v] = buf_load_cx(rU, cell[k], (4 * d + v) * g.C);
 #pragma unroll
         for (int k = 0; k < NPASS; ++k) {
             const int i = tid + k * NT;
             if (i >= BOX) continue;
-            double2 z[4];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) z[v] = buf_load_cx(rU, cell[k], (4 * d + v) * g.C);
+            const double2* z = zz[k];
 #pragma unroll
             for (int part = 0; part < 2; ++part) {
                 double du[4];
