@@ -310,6 +310,17 @@ def test_c5_residual_and_forces(c5):
     say(f"C5: residual rel {e_r:.2e}, forces rel {e_f:.2e}")
     assert e_r <= 1e-12 and e_f <= 1e-12
     assert rel(Rb[1], ref["R"]) <= 1e-12
+    # every tile shape of the batched launch (the benchmark times the shape its autotuner picks)
+    from pyeulerale_amd import _lib
+    try:
+        for t in range(_lib.load().pea_num_face_tiles()):
+            K.set_face_tile(0, g.M, g.N, t)
+            Rt = host(K.compute_odes_batch(p, c.mach, g, X, V, Ub))
+            e_t = max(rel(Rt[0], ref["R"]), rel(Rt[1], ref["R"]))
+            say(f"C5: batched residual, tile {K.face_tile(0, g.M, g.N)}: rel {e_t:.2e}")
+            assert e_t <= 1e-12, (t, e_t)
+    finally:
+        K.set_face_tile(0, g.M, g.N, -1)
 
 
 def test_c5_products_on_sampled_rows(orc, c5):
