@@ -102,8 +102,10 @@ def make_case(A, Rv, thickness, mach, aoa, alpha, alpha_dot, far, batch, seed=0)
     return X, V, U, M, N
 
 
-def graph_time_ms(torch, fn, reps=20):
-    """Average GPU time of fn() over `reps` launches captured in one graph (no host gaps)."""
+def graph_time_ms(torch, fn, reps=20, warm_ms=50.0):
+    """Average GPU time of fn() over `reps` launches captured in one graph (no host gaps), timed
+    over 3 replays after the graph has replayed for about warm_ms: the GPU's clock falls while it
+    idles (as during capture) and rises over the first ~30 ms of load."""
     fn()
     torch.cuda.synchronize()
     s = torch.cuda.Stream()
@@ -117,9 +119,13 @@ def graph_time_ms(torch, fn, reps=20):
     with torch.cuda.graph(g):
         for _ in range(reps):
             fn()
-    g.replay()
-    torch.cuda.synchronize()
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    g.replay()
+    b.record()
+    torch.cuda.synchronize()
+    for _ in range(int(np.ceil(warm_ms / max(a.elapsed_time(b), 1e-3)))):
+        g.replay()
     a.record()
     for _ in range(3):
         g.replay()
@@ -289,8 +295,13 @@ def freq_response(torch, superlu: bool, shifts: int = 32, sequential: bool = Tru
 def residual_beyond_cache(torch, K, p, mach, g, Xd, Vd, Ud, Rd, alpha, alpha_dot, reps=20):
     """The batched residual where consecutive launches cannot be served by the 256 MB Infinity
     Cache: (a) 256 states per launch (537 MB of states and residuals), (b) the 64-state launch
-    with a 1 GiB buffer written between launches, outside the timed events.  Per-launch HIP
-    events; states per second of kernel time against the 64-state launch without flushing."""
+    with a 1 GiB buffer written between launches, outside the timed events.  The 64- and
+    256-state launches are timed back to back from graphs; the flushed launches need a write
+    between launches, so they are timed one by one with HIP events, and so is the unflushed
+    64-state launch beside them (per-launch events add a few microseconds to each).  The tile
+    shape is the one the project's autotuner picks for the 64-state launch."""
+    tuned = K.autotune_face(p, mach, g, Xd, Vd, Ud[0].contiguous(), kinds=("residual",), reps=5,
+                            batch=Ud.shape[0], warmup=100)["residual"]
     def timed(fn, flush=None):
         evs = []
         for _ in range(reps):
@@ -313,16 +324,18 @@ def residual_beyond_cache(torch, K, p, mach, g, Xd, Vd, Ud, Rd, alpha, alpha_dot
     flush = lambda: junk.fill_(1.0)
     for _ in range(3):
         fn64(), fn256()
-    ms64 = timed(fn64)
+    ms64 = graph_time_ms(torch, fn64, reps)
+    ms256 = graph_time_ms(torch, fn256, reps // 2)
+    ms64e = timed(fn64)
     ms64f = timed(fn64, flush)
-    ms256 = timed(fn256)
     b64 = Ud.shape[0]
     return {"grid": "257x129", "states_per_launch": [b64, big],
-            "ms_per_launch_64": ms64, "ms_per_launch_64_cache_flushed": ms64f,
-            "ms_per_launch_256": ms256,
+            "tile": K.face_tile(0, g.M, g.N)[1:], "tile_autotuned_ms": tuned[1],
+            "ms_per_launch_64": ms64, "ms_per_launch_256": ms256,
+            "ms_per_launch_64_events": ms64e, "ms_per_launch_64_cache_flushed": ms64f,
             "states_per_s_64": b64 / (ms64 * 1e-3),
-            "states_per_s_64_cache_flushed": b64 / (ms64f * 1e-3),
-            "states_per_s_256": big / (ms256 * 1e-3)}
+            "states_per_s_256": big / (ms256 * 1e-3),
+            "cache_flush_slowdown": ms64f / ms64e}
 
 
 def profile_launches(torch, K):
