@@ -980,12 +980,6 @@ __device__ __forceinline__ void face_dirs_tile(const pea_params& P, const FsDir&
             return sU[v * BOX + b];
         };
     };
-    auto dirs = [&](int bLL, int bL, int bR, int bRR) {
-        return [bLL, bL, bR, bRR](int slot, int v) -> double {
-            const int b = slot == 0 ? bLL : slot == 1 ? bL : slot == 2 ? bR : bRR;
-            return sdU[v * BOX + b];
-        };
-    };
     auto build = [&](int bLL, int bL, int bR, int bRR, const double* nr, const double* nr2,
                      const double* gv, bool dL, bool dRs, FaceLin& f) {
         face_lin_build(P, prim(bLL, bL, bR, bRR), sP[bLL], sP[bL], sP[bR], sP[bRR], sC[bL],
@@ -1541,7 +1535,7 @@ k_forces(Grid g, const double* __restrict__ X, const double* __restrict__ U,
     red[2][threadIdx.x] = sm;
     __syncthreads();
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
+        if ((int)threadIdx.x < s)
             for (int q = 0; q < 3; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
         __syncthreads();
     }
@@ -2088,7 +2082,7 @@ __device__ __forceinline__ void face_vjp_tile(const pea_params& P, const FsDir& 
     // 3. radial faces (Lm, n) | (Lm+1, n) for the tile's columns, Lm from m0-2 to m0+TM
     {
         {
-            const int tn = fr_tn, n = fr_n, Lm = fr_Lm;
+            const int tn = fr_tn, Lm = fr_Lm;
             const bool act = fr_act;
             FaceAdj<T, L> a;
             if (act) {
@@ -2233,7 +2227,7 @@ k_sum_partials(const double* __restrict__ partial, int count, double* __restrict
     red[1][threadIdx.x] = si;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
+        if ((int)threadIdx.x < s) {
             red[0][threadIdx.x] += red[0][threadIdx.x + s];
             red[1][threadIdx.x] += red[1][threadIdx.x + s];
         }
